@@ -1,0 +1,169 @@
+"""CPU oracle for the visibility hot path (TEST INFRASTRUCTURE ONLY).
+
+Loads ``oracle/liboskar_oracle.so`` (the C restatement in oskar_oracle.c)
+and, when it has been built, ``oracle/_ref/liboskar_ref.so`` (the reference's
+own CPU kernels compiled from its sources by ``make -C oracle ref``).
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
+this package, as the checker; the product library never calls it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(_HERE, "liboskar_oracle.so")
+REF_PATH = os.path.join(_HERE, "_ref", "liboskar_ref.so")
+
+_lib = None
+_ref = None
+
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise ImportError("oracle not built: make -C oracle")
+        _lib = C.CDLL(ORACLE_PATH)
+        xc = [I, I, I, I, I] + [P] * 16 + [D, D, I, D, D, D, D, I, D, D, I, P]
+        _lib.oracle_cross_correlate_d.argtypes = xc
+        _lib.oracle_cross_correlate_f.argtypes = xc
+        dft = [I, I, I, D] + [P] * 4 + [I, I] + [P] * 5 + [I, I, I, P]
+        _lib.oracle_dftw_d.argtypes = dft
+        _lib.oracle_dftw_f.argtypes = dft
+        _lib.oracle_element_weights_dft_d.argtypes = [I] + [P] * 4 + [D] * 4 + [P]
+        _lib.oracle_element_weights_dft_f.argtypes = [I] + [P] * 4 + [D] * 4 + [P]
+        _lib.oracle_theta_phi_d.argtypes = [I, I, P, P, P, I, D, D, P, P, P]
+        _lib.oracle_dipole_pattern_d.argtypes = [I, I, P, P, P, D, D, I, I, P]
+        _lib.oracle_station_beam_aa_d.argtypes = (
+            [I, I] + [P] * 7 + [P, I, P, P, D, D, I, I, I, I, I, D, D, D, I,
+                                P, P, P, D, P])
+        _lib.oracle_station_beam_aa_d.restype = I
+        _lib.oracle_evaluate_jones_K_d.argtypes = (
+            [I, P, P, P, I, P, P, P, D, P, D, D, P])
+        _lib.oracle_factorised_subset_ld.argtypes = (
+            [I, P, P, P, P, P, I, P, P, P, D, I, P, P, P])
+    return _lib
+
+
+def ref():
+    """The reference's compiled CPU kernels, or None if not built here."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        _ref = C.CDLL(REF_PATH)
+    return _ref
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dt):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def cross_correlate(precision, use_casa, source_type, matrix, jones, I_, Q,
+                    U, V, l, m, n, a, b, c, st_u, st_v, st_w, st_x, st_y,
+                    channel_bw=0.0, time_avg=0.0, azel=False, ra0=0.0,
+                    dec0=0.0, uv_min=0.0, uv_max=-1.0, uv_wavelengths=False,
+                    gast=0.0, freq=100e6, vis=None, offset_out=0):
+    """oskar_cross_correlate semantics on host arrays; returns vis (reals,
+    accumulated into the given vis if any)."""
+    dt = np.float64 if precision == "double" else np.float32
+    ns = st_u.size
+    nb = ns * (ns - 1) // 2
+    nr = 8 if matrix else 2
+    if vis is None:
+        vis = np.zeros((offset_out + nb) * nr, dtype=dt)
+    vis = np.ascontiguousarray(vis, dtype=dt)
+    num_sources = l.size
+    arrs = [_c(x, dt) for x in (jones, I_, Q, U, V, l, m, n, a, b, c, st_u,
+                                st_v, st_w, st_x, st_y)]
+    fn = (lib().oracle_cross_correlate_d if precision == "double"
+          else lib().oracle_cross_correlate_f)
+    fn(int(use_casa), int(source_type), int(matrix), int(num_sources), ns,
+       *[_p(x) for x in arrs], float(channel_bw), float(time_avg),
+       int(bool(azel)), float(ra0), float(dec0), float(uv_min), float(uv_max),
+       int(bool(uv_wavelengths)), float(gast), float(freq), int(offset_out),
+       _p(vis))
+    return vis
+
+
+def dftw(precision, matrix, normalise, wavenumber, weights, x_in, y_in, z_in,
+         x_out, y_out, z_out, data_idx, data, eval_x=1, eval_y=1,
+         offset_coord_out=0, num_out=None, offset_out=0, output=None):
+    dt = np.float64 if precision == "double" else np.float32
+    num_in = x_in.size
+    if num_out is None:
+        num_out = x_out.size - offset_coord_out
+    nr = 8 if matrix else 2
+    if output is None:
+        output = np.zeros((offset_out + num_out) * nr, dtype=dt)
+    args = [_c(x, dt) for x in (weights, x_in, y_in, z_in)]
+    outs = [_c(x, dt) for x in (x_out, y_out, z_out)]
+    idx = _c(data_idx, np.int32)
+    d = _c(data, dt)
+    fn = lib().oracle_dftw_d if precision == "double" else lib().oracle_dftw_f
+    fn(int(matrix), int(normalise), int(num_in), float(wavenumber),
+       *[_p(x) for x in args], int(offset_coord_out), int(num_out),
+       *[_p(x) for x in outs], _p(idx), _p(d), int(eval_x), int(eval_y),
+       int(offset_out), _p(output))
+    return output
+
+
+def station_beam_aa(matrix, x_true, y_true, z_true, x_meas, y_meas, z_meas,
+                    cable, element_types, type_kind, dipole_length_m,
+                    beam_dir, x, y, z, frequency_hz, euler_x0=0.0,
+                    euler_y0=0.0, swap_xy=0, is_3d=0, norm_array=0,
+                    norm_element=0, enable_array=1):
+    f8 = np.float64
+    ne = x_true.size
+    npnt = x.size
+    beam = np.zeros(npnt * (8 if matrix else 2), dtype=f8)
+    arr = [_c(v, f8) for v in (x_true, y_true, z_true, x_meas, y_meas,
+                               z_meas, cable)]
+    et = _c(element_types, np.int32)
+    tk = _c(type_kind, np.int32)
+    dl = _c(dipole_length_m, f8)
+    xs = [_c(v, f8) for v in (x, y, z)]
+    r = lib().oracle_station_beam_aa_d(
+        int(matrix), ne, *[_p(v) for v in arr], _p(et), int(tk.size), _p(tk),
+        _p(dl), float(euler_x0), float(euler_y0), int(swap_xy), int(is_3d),
+        int(norm_array), int(norm_element), int(enable_array),
+        float(beam_dir[0]), float(beam_dir[1]), float(beam_dir[2]), npnt,
+        *[_p(v) for v in xs], float(frequency_hz), _p(beam))
+    if r != 0:
+        raise RuntimeError("oracle_station_beam_aa_d returned %d" % r)
+    return beam
+
+
+def jones_K(l, m, n, u, v, w, wavenumber, I_, fmin=-np.inf, fmax=np.inf):
+    f8 = np.float64
+    out = np.zeros(2 * u.size * l.size, dtype=f8)
+    arr = [_c(x, f8) for x in (l, m, n, u, v, w, I_)]
+    lib().oracle_evaluate_jones_K_d(
+        l.size, _p(arr[0]), _p(arr[1]), _p(arr[2]), u.size, _p(arr[3]),
+        _p(arr[4]), _p(arr[5]), float(wavenumber), _p(arr[6]), float(fmin),
+        float(fmax), _p(out))
+    return out
+
+
+def factorised_subset(l, m, n, I_, E, u, v, w, wavenumber, p, q):
+    f8 = np.float64
+    arr = [_c(x, f8) for x in (l, m, n, I_, E, u, v, w)]
+    pi = _c(p, np.int32)
+    qi = _c(q, np.int32)
+    out = np.zeros(8 * pi.size, dtype=f8)
+    lib().oracle_factorised_subset_ld(
+        l.size, _p(arr[0]), _p(arr[1]), _p(arr[2]), _p(arr[3]), _p(arr[4]),
+        u.size, _p(arr[5]), _p(arr[6]), _p(arr[7]), float(wavenumber),
+        pi.size, _p(pi), _p(qi), _p(out))
+    return out

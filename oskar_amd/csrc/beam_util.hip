@@ -1,0 +1,457 @@
+/*
+ * Element-wise kernels of the aperture-array station beam and of oskar_Mem
+ * utilities, for gfx950.
+ *
+ *  - DFT beamforming weights w_i = exp(-i k (x_i x_b + y_i y_b + z_i z_b
+ *    + cable_i)) (oskar/telescope/station/define_evaluate_element_weights_dft.h).
+ *  - ENU direction cosines -> (theta, phi_x, phi_y), with an optional extra
+ *    point at the pole (oskar/convert/define_convert_enu_directions_to_theta_phi.h).
+ *  - Half-wave dipole pattern E_theta, E_phi for the X and Y dipoles
+ *    (oskar/telescope/station/element/define_evaluate_dipole_pattern.h).
+ *  - Zeroing below the horizon (oskar/telescope/station/define_blank_below_horizon.h).
+ *  - Normalisation by one element (oskar/mem/define_mem_normalise.h).
+ *  - Set real value (oskar/mem/define_mem_set_value_real.h).
+ *  - Jones join (element-wise 2x2 complex products, oskar/mem/define_mem_multiply.h).
+ * All are one thread per element, grid-stride, 256 threads per workgroup.
+ */
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "private.h"
+
+using namespace oskar_amd;
+
+namespace {
+
+constexpr int THREADS = 256;
+
+inline dim3 grid_for(size_t n)
+{
+    size_t g = (n + THREADS - 1) / THREADS;
+    if (g > 65535u * 8u) g = 65535u * 8u;
+    if (g == 0) g = 1;
+    return dim3((unsigned) g);
+}
+
+#define GRID_STRIDE(I, N) \
+    for (size_t I = (size_t) blockIdx.x * blockDim.x + threadIdx.x; I < (N); \
+            I += (size_t) gridDim.x * blockDim.x)
+
+template <typename FP>
+__global__ void weights_kernel(int n, const FP* x, const FP* y, const FP* z,
+        const FP* cable, FP wavenumber, FP x1, FP y1, FP z1,
+        Cplx<FP>* weights)
+{
+    GRID_STRIDE(i, (size_t) n)
+    {
+        const FP p = wavenumber * (x[i] * x1 + y[i] * y1 + z[i] * z1 +
+                (cable ? cable[i] : (FP) 0));
+        FP s, c;
+        sincos_v<FP>(-p, &s, &c);
+        weights[i].x = c;
+        weights[i].y = s;
+    }
+}
+
+template <typename FP>
+__global__ void theta_phi_kernel(int off_in, int num, const FP* x,
+        const FP* y, const FP* z, int extra_point_at_pole, FP delta_phi1,
+        FP delta_phi2, FP* theta, FP* phi1, FP* phi2)
+{
+    const FP twopi = 2 * pi_v<FP>();
+    GRID_STRIDE(i, (size_t) num)
+    {
+        const FP xx = x[i + off_in], yy = y[i + off_in], zz = z[i + off_in];
+        const FP p0 = atan2(yy, xx);
+        FP p1 = fmod(p0 + delta_phi1, twopi);
+        FP p2 = fmod(p0 + delta_phi2, twopi);
+        if (p1 < (FP) 0) p1 += twopi;
+        if (p2 < (FP) 0) p2 += twopi;
+        theta[i] = atan2(sqrt(xx * xx + yy * yy), zz);
+        phi1[i] = p1;
+        phi2[i] = p2;
+    }
+    if (extra_point_at_pole && blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        theta[num] = (FP) 0;
+        phi1[num] = delta_phi1;
+        phi2[num] = delta_phi2;
+    }
+}
+
+/* One dipole: E_theta, E_phi (imaginary parts zero). */
+template <typename FP>
+__device__ __forceinline__ void dipole(FP phi, FP sin_theta, FP cos_theta,
+        FP kL, FP cos_kL, Cplx<FP>* e_theta, Cplx<FP>* e_phi)
+{
+    FP sin_phi, cos_phi;
+    sincos_v<FP>(phi, &sin_phi, &cos_phi);
+    const FP denom = (FP) 1 + cos_phi * cos_phi * (cos_theta * cos_theta -
+            (FP) 1);
+    e_theta->y = e_phi->y = (FP) 0;
+    if (denom == (FP) 0)
+    {
+        e_theta->x = e_phi->x = (FP) 0;
+    }
+    else
+    {
+        const FP t = (cos(kL * cos_phi * sin_theta) - cos_kL) / denom;
+        e_theta->x = -cos_phi * cos_theta * t;
+        e_phi->x = sin_phi * t;
+    }
+}
+
+template <typename FP, bool MATRIX>
+__global__ void dipole_kernel(int n, const FP* theta, const FP* phi_x,
+        const FP* phi_y, FP kL, FP cos_kL, int swap_xy, int offset_out,
+        Cplx<FP>* pattern)
+{
+    GRID_STRIDE(i, (size_t) n)
+    {
+        FP sin_theta, cos_theta;
+        sincos_v<FP>(theta[i], &sin_theta, &cos_theta);
+        Cplx<FP> x_th, x_ph, y_th, y_ph;
+        dipole<FP>(phi_x[i], sin_theta, cos_theta, kL, cos_kL, &x_th, &x_ph);
+        dipole<FP>(phi_y[i], sin_theta, cos_theta, kL, cos_kL, &y_th, &y_ph);
+        if (MATRIX)
+        {
+            Cplx<FP>* o = pattern + 4 * (i + offset_out);
+            if (swap_xy)
+            {
+                o[0] = y_th; o[1] = y_ph; o[2] = x_th; o[3] = x_ph;
+            }
+            else
+            {
+                o[0] = x_th; o[1] = x_ph; o[2] = y_th; o[3] = y_ph;
+            }
+        }
+        else
+        {
+            const FP amp = sqrt((x_th.x * x_th.x + x_ph.x * x_ph.x +
+                    y_th.x * y_th.x + y_ph.x * y_ph.x) / (FP) 2);
+            pattern[i + offset_out].x = amp;
+            pattern[i + offset_out].y = (FP) 0;
+        }
+    }
+}
+
+template <typename FP, int NR>
+__global__ void blank_kernel(int offset_mask, int n, const FP* mask,
+        int offset_out, FP* jones)
+{
+    GRID_STRIDE(i, (size_t) n)
+    {
+        if (mask[i + offset_mask] < (FP) 0)
+        {
+            FP* o = jones + (size_t) (offset_out + i) * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) o[r] = (FP) 0;
+        }
+    }
+}
+
+/* NR reals per element: 1 real, 2 complex, 8 matrix. */
+template <typename FP, int NR>
+__global__ void normalise_kernel(unsigned int offset, unsigned int n,
+        FP* a, unsigned int idx)
+{
+    const FP* v = a + (size_t) idx * NR;
+    FP scal;
+    if (NR == 1) scal = (FP) 1 / v[0];
+    else
+    {
+        FP amp = (FP) 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) amp += v[r] * v[r];
+        if (NR == 8) amp /= (FP) 2;
+        scal = (FP) 1 / sqrt(amp);
+    }
+    GRID_STRIDE(i, (size_t) n)
+    {
+        const size_t j = offset + i;
+        if (j != idx)
+        {
+            FP* o = a + j * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) o[r] *= scal;
+        }
+    }
+}
+
+template <typename FP, int NR>
+__global__ void set_value_kernel(size_t offset, size_t n, FP value, FP* a)
+{
+    GRID_STRIDE(i, n)
+    {
+        FP* o = a + (offset + i) * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) o[r] = (FP) 0;
+        o[0] = value;
+        if (NR == 8) o[6] = value;
+    }
+}
+
+template <typename FP>
+__device__ __forceinline__ Cplx<FP> cmul(Cplx<FP> a, Cplx<FP> b)
+{
+    Cplx<FP> c;
+    c.x = a.x * b.x - a.y * b.y;
+    c.y = a.x * b.y + a.y * b.x;
+    return c;
+}
+
+/* out = a * b per element; a scalar operand is promoted to a*I. */
+template <typename FP, bool AM, bool BM>
+__global__ void join_kernel(size_t n, Cplx<FP>* out, const Cplx<FP>* a,
+        const Cplx<FP>* b)
+{
+    GRID_STRIDE(i, n)
+    {
+        if (!AM && !BM)
+        {
+            out[i] = cmul(a[i], b[i]);
+        }
+        else if (!AM)
+        {
+            const Cplx<FP> s = a[i];
+            Cplx<FP> r[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = cmul(s, b[4 * i + k]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) out[4 * i + k] = r[k];
+        }
+        else if (!BM)
+        {
+            const Cplx<FP> s = b[i];
+            Cplx<FP> r[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = cmul(a[4 * i + k], s);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) out[4 * i + k] = r[k];
+        }
+        else
+        {
+            const Cplx<FP>* A = a + 4 * i;
+            const Cplx<FP>* B = b + 4 * i;
+            Cplx<FP> r[4];
+            /* [[a b],[c d]] x [[e f],[g h]] */
+            Cplx<FP> t0 = cmul(A[0], B[0]), t1 = cmul(A[1], B[2]);
+            r[0].x = t0.x + t1.x; r[0].y = t0.y + t1.y;
+            t0 = cmul(A[0], B[1]); t1 = cmul(A[1], B[3]);
+            r[1].x = t0.x + t1.x; r[1].y = t0.y + t1.y;
+            t0 = cmul(A[2], B[0]); t1 = cmul(A[3], B[2]);
+            r[2].x = t0.x + t1.x; r[2].y = t0.y + t1.y;
+            t0 = cmul(A[2], B[1]); t1 = cmul(A[3], B[3]);
+            r[3].x = t0.x + t1.x; r[3].y = t0.y + t1.y;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) out[4 * i + k] = r[k];
+        }
+    }
+}
+
+} /* namespace */
+
+extern "C" int oskar_amd_element_weights_dft_launch(int precision,
+        int num_elements, const void* x, const void* y, const void* z,
+        const void* cable, double wavenumber, double x1, double y1, double z1,
+        void* weights, void* stream)
+{
+    if (num_elements <= 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+    if (precision == OSKAR_DOUBLE)
+        hipLaunchKernelGGL(weights_kernel<double>, grid_for(num_elements),
+                dim3(THREADS), 0, s, num_elements, (const double*) x,
+                (const double*) y, (const double*) z, (const double*) cable,
+                wavenumber, x1, y1, z1, (Cplx<double>*) weights);
+    else
+        hipLaunchKernelGGL(weights_kernel<float>, grid_for(num_elements),
+                dim3(THREADS), 0, s, num_elements, (const float*) x,
+                (const float*) y, (const float*) z, (const float*) cable,
+                (float) wavenumber, (float) x1, (float) y1, (float) z1,
+                (Cplx<float>*) weights);
+    return (int) hipGetLastError();
+}
+
+extern "C" int oskar_amd_theta_phi_launch(int precision, int offset_in,
+        int num, const void* x, const void* y, const void* z,
+        int extra_point_at_pole, double delta_phi1, double delta_phi2,
+        void* theta, void* phi1, void* phi2, void* stream)
+{
+    const hipStream_t s = (hipStream_t) stream;
+    if (precision == OSKAR_DOUBLE)
+        hipLaunchKernelGGL(theta_phi_kernel<double>, grid_for(num),
+                dim3(THREADS), 0, s, offset_in, num, (const double*) x,
+                (const double*) y, (const double*) z, extra_point_at_pole,
+                delta_phi1, delta_phi2, (double*) theta, (double*) phi1,
+                (double*) phi2);
+    else
+        hipLaunchKernelGGL(theta_phi_kernel<float>, grid_for(num),
+                dim3(THREADS), 0, s, offset_in, num, (const float*) x,
+                (const float*) y, (const float*) z, extra_point_at_pole,
+                (float) delta_phi1, (float) delta_phi2, (float*) theta,
+                (float*) phi1, (float*) phi2);
+    return (int) hipGetLastError();
+}
+
+extern "C" int oskar_amd_dipole_launch(int precision, int matrix,
+        int num_points, const void* theta, const void* phi_x,
+        const void* phi_y, double kL, double cos_kL, int swap_xy,
+        int offset_out, void* pattern, void* stream)
+{
+    if (num_points <= 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+    if (precision == OSKAR_DOUBLE)
+    {
+        if (matrix)
+            hipLaunchKernelGGL((dipole_kernel<double, true>),
+                    grid_for(num_points), dim3(THREADS), 0, s, num_points,
+                    (const double*) theta, (const double*) phi_x,
+                    (const double*) phi_y, kL, cos_kL, swap_xy, offset_out,
+                    (Cplx<double>*) pattern);
+        else
+            hipLaunchKernelGGL((dipole_kernel<double, false>),
+                    grid_for(num_points), dim3(THREADS), 0, s, num_points,
+                    (const double*) theta, (const double*) phi_x,
+                    (const double*) phi_y, kL, cos_kL, swap_xy, offset_out,
+                    (Cplx<double>*) pattern);
+    }
+    else
+    {
+        if (matrix)
+            hipLaunchKernelGGL((dipole_kernel<float, true>),
+                    grid_for(num_points), dim3(THREADS), 0, s, num_points,
+                    (const float*) theta, (const float*) phi_x,
+                    (const float*) phi_y, (float) kL, (float) cos_kL, swap_xy,
+                    offset_out, (Cplx<float>*) pattern);
+        else
+            hipLaunchKernelGGL((dipole_kernel<float, false>),
+                    grid_for(num_points), dim3(THREADS), 0, s, num_points,
+                    (const float*) theta, (const float*) phi_x,
+                    (const float*) phi_y, (float) kL, (float) cos_kL, swap_xy,
+                    offset_out, (Cplx<float>*) pattern);
+    }
+    return (int) hipGetLastError();
+}
+
+extern "C" int oskar_amd_blank_below_horizon_launch(int precision,
+        int matrix, int offset_mask, int num, const void* mask,
+        int offset_out, void* jones, void* stream)
+{
+    if (num <= 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+    if (precision == OSKAR_DOUBLE)
+    {
+        if (matrix)
+            hipLaunchKernelGGL((blank_kernel<double, 8>), grid_for(num),
+                    dim3(THREADS), 0, s, offset_mask, num,
+                    (const double*) mask, offset_out, (double*) jones);
+        else
+            hipLaunchKernelGGL((blank_kernel<double, 2>), grid_for(num),
+                    dim3(THREADS), 0, s, offset_mask, num,
+                    (const double*) mask, offset_out, (double*) jones);
+    }
+    else
+    {
+        if (matrix)
+            hipLaunchKernelGGL((blank_kernel<float, 8>), grid_for(num),
+                    dim3(THREADS), 0, s, offset_mask, num,
+                    (const float*) mask, offset_out, (float*) jones);
+        else
+            hipLaunchKernelGGL((blank_kernel<float, 2>), grid_for(num),
+                    dim3(THREADS), 0, s, offset_mask, num,
+                    (const float*) mask, offset_out, (float*) jones);
+    }
+    return (int) hipGetLastError();
+}
+
+extern "C" int oskar_amd_mem_normalise_launch(int type, unsigned int offset,
+        unsigned int n, void* data, unsigned int idx, void* stream)
+{
+    if (n == 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+    switch (type)
+    {
+    case OSKAR_SINGLE:
+        hipLaunchKernelGGL((normalise_kernel<float, 1>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, (float*) data, idx); break;
+    case OSKAR_DOUBLE:
+        hipLaunchKernelGGL((normalise_kernel<double, 1>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, (double*) data, idx); break;
+    case OSKAR_SINGLE_COMPLEX:
+        hipLaunchKernelGGL((normalise_kernel<float, 2>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, (float*) data, idx); break;
+    case OSKAR_DOUBLE_COMPLEX:
+        hipLaunchKernelGGL((normalise_kernel<double, 2>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, (double*) data, idx); break;
+    case OSKAR_SINGLE_COMPLEX_MATRIX:
+        hipLaunchKernelGGL((normalise_kernel<float, 8>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, (float*) data, idx); break;
+    case OSKAR_DOUBLE_COMPLEX_MATRIX:
+        hipLaunchKernelGGL((normalise_kernel<double, 8>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, (double*) data, idx); break;
+    default:
+        return OSKAR_ERR_BAD_DATA_TYPE;
+    }
+    return (int) hipGetLastError();
+}
+
+extern "C" int oskar_amd_mem_set_value_real_launch(int type, size_t offset,
+        size_t n, double value, void* data, void* stream)
+{
+    if (n == 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+    switch (type)
+    {
+    case OSKAR_SINGLE:
+        hipLaunchKernelGGL((set_value_kernel<float, 1>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, (float) value, (float*) data);
+        break;
+    case OSKAR_DOUBLE:
+        hipLaunchKernelGGL((set_value_kernel<double, 1>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, value, (double*) data); break;
+    case OSKAR_SINGLE_COMPLEX:
+        hipLaunchKernelGGL((set_value_kernel<float, 2>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, (float) value, (float*) data);
+        break;
+    case OSKAR_DOUBLE_COMPLEX:
+        hipLaunchKernelGGL((set_value_kernel<double, 2>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, value, (double*) data); break;
+    case OSKAR_SINGLE_COMPLEX_MATRIX:
+        hipLaunchKernelGGL((set_value_kernel<float, 8>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, (float) value, (float*) data);
+        break;
+    case OSKAR_DOUBLE_COMPLEX_MATRIX:
+        hipLaunchKernelGGL((set_value_kernel<double, 8>), grid_for(n),
+                dim3(THREADS), 0, s, offset, n, value, (double*) data); break;
+    default:
+        return OSKAR_ERR_BAD_DATA_TYPE;
+    }
+    return (int) hipGetLastError();
+}
+
+extern "C" int oskar_amd_jones_join_launch(int precision, int a_matrix,
+        int b_matrix, size_t n, void* out, const void* a, const void* b,
+        void* stream)
+{
+    if (n == 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+#define JOIN(FP, AM, BM) hipLaunchKernelGGL((join_kernel<FP, AM, BM>),       \
+        grid_for(n), dim3(THREADS), 0, s, n, (Cplx<FP>*) out,                \
+        (const Cplx<FP>*) a, (const Cplx<FP>*) b)
+    if (precision == OSKAR_DOUBLE)
+    {
+        if (a_matrix && b_matrix) JOIN(double, true, true);
+        else if (a_matrix) JOIN(double, true, false);
+        else if (b_matrix) JOIN(double, false, true);
+        else JOIN(double, false, false);
+    }
+    else
+    {
+        if (a_matrix && b_matrix) JOIN(float, true, true);
+        else if (a_matrix) JOIN(float, true, false);
+        else if (b_matrix) JOIN(float, false, true);
+        else JOIN(float, false, false);
+    }
+#undef JOIN
+    return (int) hipGetLastError();
+}

@@ -1,0 +1,188 @@
+/*
+ * Private structures and internal launch interfaces of the library.
+ * Host code is C99; kernels live in the .hip files and are reached through
+ * the extern "C" launchers declared at the bottom of this file.
+ */
+#ifndef OSKAR_AMD_PRIVATE_H_
+#define OSKAR_AMD_PRIVATE_H_
+
+#include "oskar_global.h"
+#include "oskar_mem.h"
+#include "oskar_jones.h"
+#include "oskar_telescope.h"
+#include "oskar_station.h"
+#include "oskar_timer.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* oskar/mem/private_mem.h:24-38 (the mutex is dropped: reference counting is
+ * only touched from the owning host thread in this library). */
+struct oskar_Mem
+{
+    int type;
+    int location;
+    size_t num_elements;
+    int owner;
+    void* data;
+    int ref_count;
+};
+
+/* oskar/interferometer/private_jones.h:34-41 */
+struct oskar_Jones
+{
+    int num_stations;
+    int num_sources;
+    int cap_stations;
+    int cap_sources;
+    oskar_Mem* data;
+};
+
+/* The fields of oskar/telescope/private_telescope.h:13-68 that the
+ * correlator reads. */
+struct oskar_Telescope
+{
+    int precision;
+    int mem_location;
+    int num_stations;
+    int phase_centre_coord_type;
+    double phase_centre_rad[2];
+    double channel_bandwidth_hz;
+    double time_average_sec;
+    double uv_filter_min;
+    double uv_filter_max;
+    int uv_filter_units;
+    oskar_Mem* station_true_offset_ecef_metres[3];
+};
+
+/* oskar/telescope/station/element/private_element.h (dipole/isotropic). */
+struct oskar_Element
+{
+    int precision;
+    int mem_location;
+    int element_type;
+    double dipole_length;
+    int dipole_length_units;
+};
+
+/* Aperture-array subset of oskar/telescope/station/private_station.h. */
+struct oskar_Station
+{
+    int precision;
+    int mem_location;
+    int station_type;
+    int normalise_final_beam;
+    double offset_ecef[3];
+    double lon_rad, lat_rad, alt_metres;
+    double beam_lon_rad, beam_lat_rad;
+    int beam_coord_type;
+    int num_elements;
+    int num_element_types;
+    int normalise_array_pattern;
+    int normalise_element_pattern;
+    int enable_array_pattern;
+    int common_element_orientation;
+    int common_pol_beams;
+    int swap_xy;
+    int array_is_3d;
+    oskar_Mem* element_true_enu_metres[3];
+    oskar_Mem* element_measured_enu_metres[3];
+    oskar_Mem* element_cable_length_error;
+    oskar_Mem* element_euler_cpu[2][3];
+    oskar_Mem* element_types;     /* int, station location */
+    oskar_Mem* element_types_cpu; /* int, host */
+    oskar_Element** element;
+};
+
+/* oskar/telescope/station/private_station_work.h (subset). */
+struct oskar_StationWork
+{
+    int precision;
+    int mem_location;
+    oskar_Mem* weights;
+    oskar_Mem* signal;
+    oskar_Mem* theta;
+    oskar_Mem* phi_x;
+    oskar_Mem* phi_y;
+};
+
+struct oskar_Timer
+{
+    int type;
+    int paused;
+    double elapsed;   /* accumulated seconds */
+    double start;     /* host wall clock at last resume */
+    void* ev_start;   /* hipEvent_t */
+    void* ev_end;     /* hipEvent_t */
+};
+
+/* ---- internal helpers (host.c) ---------------------------------------- */
+void* oskar_amd_current_stream(void);
+void oskar_amd_check_launch(int* status);
+int oskar_amd_hip_status(int hip_error);
+void oskar_amd_set_last_error(int error);
+
+/* ---- kernel launchers (extern "C" in the .hip files) ------------------ */
+/* All return a hipError_t value (0 on success) and launch on `stream`. */
+
+/* xcorr_direct.hip */
+typedef struct oskar_XcorrParams
+{
+    int use_casa;
+    int num_sources;
+    int num_stations;
+    int offset_out;
+    int gaussian;
+    int bandwidth_smearing;
+    int time_smearing;
+    double uv_min_lambda, uv_max_lambda;
+    double inv_wavelength, frac_bandwidth;
+    double time_int_sec, gha0_rad, dec0_rad;
+} oskar_XcorrParams;
+
+int oskar_amd_xcorr_launch(int precision, int matrix,
+        const oskar_XcorrParams* p, const void* jones,
+        const void* I, const void* Q, const void* U, const void* V,
+        const void* l, const void* m, const void* n,
+        const void* a, const void* b, const void* c,
+        const void* st_u, const void* st_v, const void* st_w,
+        const void* st_x, const void* st_y, void* vis, void* stream);
+
+/* dftw.hip */
+int oskar_amd_dftw_launch(int precision, int matrix, int is_3d,
+        int num_in, double wavenumber, const void* weights,
+        const void* x_in, const void* y_in, const void* z_in,
+        int offset_coord_out, int num_out, const void* x_out,
+        const void* y_out, const void* z_out, const int* data_idx,
+        const void* data, int num_rows, int eval_x, int eval_y,
+        int offset_out, void* output, double norm_factor, void* stream);
+
+/* beam_util.hip */
+int oskar_amd_element_weights_dft_launch(int precision, int num_elements,
+        const void* x, const void* y, const void* z, const void* cable,
+        double wavenumber, double x1, double y1, double z1, void* weights,
+        void* stream);
+int oskar_amd_theta_phi_launch(int precision, int offset_in, int num,
+        const void* x, const void* y, const void* z, int extra_point_at_pole,
+        double delta_phi1, double delta_phi2, void* theta, void* phi1,
+        void* phi2, void* stream);
+int oskar_amd_dipole_launch(int precision, int matrix, int num_points,
+        const void* theta, const void* phi_x, const void* phi_y, double kL,
+        double cos_kL, int swap_xy, int offset_out, void* pattern,
+        void* stream);
+int oskar_amd_blank_below_horizon_launch(int precision, int matrix,
+        int offset_mask, int num, const void* mask, int offset_out,
+        void* jones, void* stream);
+int oskar_amd_mem_normalise_launch(int type, unsigned int offset,
+        unsigned int n, void* data, unsigned int idx, void* stream);
+int oskar_amd_mem_set_value_real_launch(int type, size_t offset, size_t n,
+        double value, void* data, void* stream);
+int oskar_amd_jones_join_launch(int precision, int a_matrix, int b_matrix,
+        size_t n, void* out, const void* a, const void* b, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OSKAR_AMD_PRIVATE_H_ */

@@ -1,0 +1,791 @@
+/*
+ * Aperture-array station model, element model, work buffers and the
+ * station-beam evaluation chain.
+ *
+ * Reference:
+ *  - oskar/telescope/station/src/oskar_station_create.c:20-120 (defaults),
+ *    oskar_station_set_element_coords.c, oskar_station_analyse.c:20-175,
+ *    oskar_station_set_element_feed_angle.c:28-47;
+ *  - oskar/telescope/station/src/oskar_evaluate_station_beam_aperture_array.c
+ *    :70-240 (element patterns per type into a signal buffer of
+ *    num_types x (num_points + 1), weights per feed, oskar_dftw over the true
+ *    element positions with the element types as data indices, blanking);
+ *  - oskar/telescope/station/element/src/oskar_element_evaluate.c:22-170
+ *    (theta/phi with delta_phi = pi/2 - orientation, dipole or isotropic,
+ *    optional normalisation by an extra point at the pole);
+ *  - oskar/telescope/station/src/oskar_station_evaluate_element_weights.c:15-66;
+ *  - oskar/telescope/station/src/oskar_station_beam_horizon_direction.c:17-90;
+ *  - oskar/convert/define_convert_apparent_ra_dec_to_enu_directions.h.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "private.h"
+#include "oskar_dftw.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#define C_0 299792458.0
+
+/* ---- Element ------------------------------------------------------------ */
+static oskar_Element* element_create(int precision, int location)
+{
+    oskar_Element* e = (oskar_Element*) calloc(1, sizeof(oskar_Element));
+    if (!e) return 0;
+    e->precision = precision;
+    e->mem_location = location;
+    e->element_type = OSKAR_ELEMENT_TYPE_DIPOLE;
+    e->dipole_length = 0.5;
+    e->dipole_length_units = OSKAR_WAVELENGTHS;
+    return e;
+}
+
+void oskar_element_set_element_type(oskar_Element* data, const char* type,
+        int* status)
+{
+    if (*status || !data) return;
+    if (!type || !type[0]) { *status = OSKAR_ERR_INVALID_ARGUMENT; return; }
+    if (type[0] == 'D' || type[0] == 'd')
+        data->element_type = OSKAR_ELEMENT_TYPE_DIPOLE;
+    else if (type[0] == 'I' || type[0] == 'i')
+        data->element_type = OSKAR_ELEMENT_TYPE_ISOTROPIC;
+    else *status = OSKAR_ERR_INVALID_ARGUMENT;
+}
+
+void oskar_element_set_dipole_length(oskar_Element* data, double value,
+        const char* units, int* status)
+{
+    if (*status || !data) return;
+    if (!units || !units[0]) { *status = OSKAR_ERR_BAD_UNITS; return; }
+    if (units[0] == 'M' || units[0] == 'm')
+        data->dipole_length_units = OSKAR_METRES;
+    else if (units[0] == 'W' || units[0] == 'w')
+        data->dipole_length_units = OSKAR_WAVELENGTHS;
+    else { *status = OSKAR_ERR_BAD_UNITS; return; }
+    data->dipole_length = value;
+}
+
+int oskar_element_type(const oskar_Element* d) { return d->element_type; }
+double oskar_element_dipole_length(const oskar_Element* d)
+{ return d->dipole_length; }
+int oskar_element_dipole_length_units(const oskar_Element* d)
+{ return d->dipole_length_units; }
+
+/* ---- Station ------------------------------------------------------------ */
+oskar_Station* oskar_station_create(int type, int location, int num_elements,
+        int* status)
+{
+    int i = 0, f = 0;
+    oskar_Station* s = 0;
+    if (*status) return 0;
+    if (type != OSKAR_SINGLE && type != OSKAR_DOUBLE)
+    {
+        *status = OSKAR_ERR_BAD_DATA_TYPE;
+        return 0;
+    }
+    if (location != OSKAR_CPU && location != OSKAR_GPU)
+    {
+        *status = OSKAR_ERR_BAD_LOCATION;
+        return 0;
+    }
+    s = (oskar_Station*) calloc(1, sizeof(oskar_Station));
+    if (!s) { *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE; return 0; }
+    s->precision = type;
+    s->mem_location = location;
+    s->station_type = OSKAR_STATION_TYPE_AA;
+    s->normalise_final_beam = OSKAR_TRUE;
+    s->beam_coord_type = OSKAR_COORDS_RADEC;
+    s->num_elements = num_elements;
+    s->enable_array_pattern = OSKAR_TRUE;
+    s->common_element_orientation = OSKAR_TRUE;
+    s->common_pol_beams = OSKAR_TRUE;
+    for (i = 0; i < 3; ++i)
+    {
+        s->element_true_enu_metres[i] =
+                oskar_mem_create(type, location, num_elements, status);
+        s->element_measured_enu_metres[i] =
+                oskar_mem_create(type, location, num_elements, status);
+    }
+    s->element_cable_length_error =
+            oskar_mem_create(type, location, num_elements, status);
+    for (f = 0; f < 2; ++f)
+        for (i = 0; i < 3; ++i)
+            s->element_euler_cpu[f][i] = oskar_mem_create(OSKAR_DOUBLE,
+                    OSKAR_CPU, num_elements, status);
+    s->element_types = oskar_mem_create(OSKAR_INT, location, num_elements,
+            status);
+    s->element_types_cpu = oskar_mem_create(OSKAR_INT, OSKAR_CPU,
+            num_elements, status);
+    oskar_mem_clear_contents(s->element_cable_length_error, status);
+    oskar_mem_clear_contents(s->element_types, status);
+    for (i = 0; i < 3; ++i)
+    {
+        oskar_mem_clear_contents(s->element_true_enu_metres[i], status);
+        oskar_mem_clear_contents(s->element_measured_enu_metres[i], status);
+    }
+    oskar_station_resize_element_types(s, 1, status);
+    return s;
+}
+
+void oskar_station_free(oskar_Station* s, int* status)
+{
+    int i = 0, f = 0;
+    if (!s) return;
+    for (i = 0; i < 3; ++i)
+    {
+        oskar_mem_free(s->element_true_enu_metres[i], status);
+        oskar_mem_free(s->element_measured_enu_metres[i], status);
+    }
+    oskar_mem_free(s->element_cable_length_error, status);
+    for (f = 0; f < 2; ++f)
+        for (i = 0; i < 3; ++i) oskar_mem_free(s->element_euler_cpu[f][i],
+                status);
+    oskar_mem_free(s->element_types, status);
+    oskar_mem_free(s->element_types_cpu, status);
+    for (i = 0; i < s->num_element_types; ++i) free(s->element[i]);
+    free(s->element);
+    free(s);
+}
+
+static void resize_zero(oskar_Mem* m, size_t n, int* status)
+{
+    const size_t old = oskar_mem_length(m);
+    oskar_mem_realloc(m, n, status);
+    if (*status || n <= old) return;
+    /* Clear the new tail through a temporary host buffer. */
+    {
+        oskar_Mem* z = oskar_mem_create(oskar_mem_type(m), OSKAR_CPU,
+                n - old, status);
+        oskar_mem_copy_contents(m, z, old, 0, n - old, status);
+        oskar_mem_free(z, status);
+    }
+}
+
+void oskar_station_resize(oskar_Station* s, int num_elements, int* status)
+{
+    int i = 0, f = 0;
+    if (*status || !s) return;
+    for (i = 0; i < 3; ++i)
+    {
+        resize_zero(s->element_true_enu_metres[i], num_elements, status);
+        resize_zero(s->element_measured_enu_metres[i], num_elements, status);
+    }
+    resize_zero(s->element_cable_length_error, num_elements, status);
+    for (f = 0; f < 2; ++f)
+        for (i = 0; i < 3; ++i)
+            resize_zero(s->element_euler_cpu[f][i], num_elements, status);
+    resize_zero(s->element_types, num_elements, status);
+    resize_zero(s->element_types_cpu, num_elements, status);
+    s->num_elements = num_elements;
+}
+
+oskar_Station* oskar_station_create_copy(const oskar_Station* src,
+        int location, int* status)
+{
+    int i = 0, f = 0;
+    oskar_Station* s = 0;
+    if (*status || !src) return 0;
+    s = oskar_station_create(src->precision, location, src->num_elements,
+            status);
+    if (*status) return s;
+    s->station_type = src->station_type;
+    s->normalise_final_beam = src->normalise_final_beam;
+    memcpy(s->offset_ecef, src->offset_ecef, sizeof(src->offset_ecef));
+    s->lon_rad = src->lon_rad;
+    s->lat_rad = src->lat_rad;
+    s->alt_metres = src->alt_metres;
+    s->beam_lon_rad = src->beam_lon_rad;
+    s->beam_lat_rad = src->beam_lat_rad;
+    s->beam_coord_type = src->beam_coord_type;
+    s->normalise_array_pattern = src->normalise_array_pattern;
+    s->normalise_element_pattern = src->normalise_element_pattern;
+    s->enable_array_pattern = src->enable_array_pattern;
+    s->common_element_orientation = src->common_element_orientation;
+    s->common_pol_beams = src->common_pol_beams;
+    s->swap_xy = src->swap_xy;
+    s->array_is_3d = src->array_is_3d;
+    for (i = 0; i < 3; ++i)
+    {
+        oskar_mem_copy(s->element_true_enu_metres[i],
+                src->element_true_enu_metres[i], status);
+        oskar_mem_copy(s->element_measured_enu_metres[i],
+                src->element_measured_enu_metres[i], status);
+    }
+    oskar_mem_copy(s->element_cable_length_error,
+            src->element_cable_length_error, status);
+    for (f = 0; f < 2; ++f)
+        for (i = 0; i < 3; ++i)
+            oskar_mem_copy(s->element_euler_cpu[f][i],
+                    src->element_euler_cpu[f][i], status);
+    oskar_mem_copy(s->element_types, src->element_types, status);
+    oskar_mem_copy(s->element_types_cpu, src->element_types_cpu, status);
+    oskar_station_resize_element_types(s, src->num_element_types, status);
+    for (i = 0; i < src->num_element_types && !*status; ++i)
+        *s->element[i] = *src->element[i];
+    return s;
+}
+
+void oskar_station_resize_element_types(oskar_Station* s,
+        int num_element_types, int* status)
+{
+    int i = 0;
+    if (*status || !s) return;
+    if (num_element_types < s->num_element_types)
+    {
+        for (i = num_element_types; i < s->num_element_types; ++i)
+            free(s->element[i]);
+    }
+    {
+        oskar_Element** e = (oskar_Element**) realloc(s->element,
+                (num_element_types > 0 ? num_element_types : 1) *
+                sizeof(oskar_Element*));
+        if (!e) { *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE; return; }
+        s->element = e;
+    }
+    for (i = s->num_element_types; i < num_element_types; ++i)
+    {
+        s->element[i] = element_create(s->precision, s->mem_location);
+        if (!s->element[i]) { *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE; }
+    }
+    s->num_element_types = num_element_types;
+}
+
+static void set_real(oskar_Mem* m, int index, double v, int* status)
+{
+    oskar_mem_set_element_real(m, (size_t) index, v, status);
+}
+
+void oskar_station_set_element_coords(oskar_Station* s, int feed, int index,
+        const double measured_enu[3], const double true_enu[3], int* status)
+{
+    int i = 0;
+    if (*status || !s) return;
+    if (index < 0 || index >= s->num_elements || feed != 0)
+    {
+        /* Only one feed of coordinates (shared by both polarisations). */
+        *status = (feed != 0) ? OSKAR_ERR_FUNCTION_NOT_AVAILABLE :
+                OSKAR_ERR_OUT_OF_RANGE;
+        return;
+    }
+    for (i = 0; i < 3; ++i)
+    {
+        set_real(s->element_measured_enu_metres[i], index, measured_enu[i],
+                status);
+        set_real(s->element_true_enu_metres[i], index, true_enu[i], status);
+    }
+    if (measured_enu[2] != 0.0 || true_enu[2] != 0.0) s->array_is_3d = 1;
+}
+
+void oskar_station_set_element_cable_length_error(oskar_Station* s, int feed,
+        int index, const double error_metres, int* status)
+{
+    if (*status || !s) return;
+    if (feed != 0) { *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE; return; }
+    if (index < 0 || index >= s->num_elements)
+    {
+        *status = OSKAR_ERR_OUT_OF_RANGE;
+        return;
+    }
+    set_real(s->element_cable_length_error, index, error_metres, status);
+}
+
+void oskar_station_set_element_feed_angle(oskar_Station* s, int feed,
+        int index, double alpha_deg, double beta_deg, double gamma_deg,
+        int* status)
+{
+    const double deg2rad = M_PI / 180.0;
+    if (*status || !s) return;
+    if (index < 0 || index >= s->num_elements || feed < 0 || feed > 1)
+    {
+        *status = OSKAR_ERR_OUT_OF_RANGE;
+        return;
+    }
+    ((double*) oskar_mem_void(s->element_euler_cpu[feed][0]))[index] =
+            alpha_deg * deg2rad;
+    ((double*) oskar_mem_void(s->element_euler_cpu[feed][1]))[index] =
+            beta_deg * deg2rad;
+    ((double*) oskar_mem_void(s->element_euler_cpu[feed][2]))[index] =
+            gamma_deg * deg2rad;
+}
+
+void oskar_station_set_element_type(oskar_Station* s, int index,
+        int element_type, int* status)
+{
+    if (*status || !s) return;
+    if (index < 0 || index >= s->num_elements)
+    {
+        *status = OSKAR_ERR_OUT_OF_RANGE;
+        return;
+    }
+    ((int*) oskar_mem_void(s->element_types_cpu))[index] = element_type;
+    set_real(s->element_types, index, (double) element_type, status);
+}
+
+void oskar_station_analyse(oskar_Station* s,
+        int* finished_identical_station_check, int* status)
+{
+    int f = 0, i = 0, d = 0;
+    (void) finished_identical_station_check;
+    if (*status || !s) return;
+    s->array_is_3d = 0;
+    s->common_element_orientation = 1;
+    s->common_pol_beams = 1;
+    for (f = 0; f < 2; ++f)
+    {
+        const double* e[3];
+        for (d = 0; d < 3; ++d)
+            e[d] = (const double*) oskar_mem_void_const(
+                    s->element_euler_cpu[f][d]);
+        for (i = 1; i < s->num_elements; ++i)
+            if (e[0][i] != e[0][0] || e[1][i] != e[1][0] || e[2][i] != e[2][0])
+            {
+                s->common_element_orientation = 0;
+                break;
+            }
+    }
+    {
+        const oskar_Mem* z[2] = { s->element_true_enu_metres[2],
+                s->element_measured_enu_metres[2] };
+        for (d = 0; d < 2 && !*status; ++d)
+        {
+            oskar_Mem* h = oskar_mem_create_copy(z[d], OSKAR_CPU, status);
+            for (i = 0; i < s->num_elements && !*status; ++i)
+            {
+                const double v = oskar_mem_get_element(h, (size_t) i, status);
+                if (v != 0.0) { s->array_is_3d = 1; break; }
+            }
+            oskar_mem_free(h, status);
+        }
+    }
+    /* Keep the device copy of the element types in step with the host. */
+    oskar_mem_copy(s->element_types, s->element_types_cpu, status);
+}
+
+void oskar_station_set_station_type(oskar_Station* m, int type)
+{ m->station_type = type; }
+void oskar_station_set_normalise_final_beam(oskar_Station* m, int v)
+{ m->normalise_final_beam = v; }
+void oskar_station_set_position(oskar_Station* m, double lon, double lat,
+        double alt, double x, double y, double z)
+{
+    m->lon_rad = lon; m->lat_rad = lat; m->alt_metres = alt;
+    m->offset_ecef[0] = x; m->offset_ecef[1] = y; m->offset_ecef[2] = z;
+}
+void oskar_station_set_phase_centre(oskar_Station* m, int coord_type,
+        double lon, double lat)
+{
+    m->beam_coord_type = coord_type;
+    m->beam_lon_rad = lon;
+    m->beam_lat_rad = lat;
+}
+void oskar_station_set_normalise_array_pattern(oskar_Station* m, int v)
+{ m->normalise_array_pattern = v; }
+void oskar_station_set_normalise_element_pattern(oskar_Station* m, int v)
+{ m->normalise_element_pattern = v; }
+void oskar_station_set_enable_array_pattern(oskar_Station* m, int v)
+{ m->enable_array_pattern = v; }
+void oskar_station_set_swap_xy(oskar_Station* m, int v) { m->swap_xy = v; }
+
+int oskar_station_precision(const oskar_Station* m) { return m->precision; }
+int oskar_station_mem_location(const oskar_Station* m)
+{ return m->mem_location; }
+int oskar_station_type(const oskar_Station* m) { return m->station_type; }
+int oskar_station_num_elements(const oskar_Station* m)
+{ return m->num_elements; }
+int oskar_station_num_element_types(const oskar_Station* m)
+{ return m->num_element_types; }
+int oskar_station_normalise_final_beam(const oskar_Station* m)
+{ return m->normalise_final_beam; }
+int oskar_station_normalise_array_pattern(const oskar_Station* m)
+{ return m->normalise_array_pattern; }
+int oskar_station_normalise_element_pattern(const oskar_Station* m)
+{ return m->normalise_element_pattern; }
+int oskar_station_enable_array_pattern(const oskar_Station* m)
+{ return m->enable_array_pattern; }
+int oskar_station_swap_xy(const oskar_Station* m) { return m->swap_xy; }
+int oskar_station_array_is_3d(const oskar_Station* m)
+{ return m->array_is_3d; }
+int oskar_station_common_element_orientation(const oskar_Station* m)
+{ return m->common_element_orientation; }
+int oskar_station_common_pol_beams(const oskar_Station* m)
+{ return m->common_pol_beams; }
+double oskar_station_lon_rad(const oskar_Station* m) { return m->lon_rad; }
+double oskar_station_lat_rad(const oskar_Station* m) { return m->lat_rad; }
+double oskar_station_beam_lon_rad(const oskar_Station* m)
+{ return m->beam_lon_rad; }
+double oskar_station_beam_lat_rad(const oskar_Station* m)
+{ return m->beam_lat_rad; }
+int oskar_station_beam_coord_type(const oskar_Station* m)
+{ return m->beam_coord_type; }
+oskar_Mem* oskar_station_element_true_enu_metres(oskar_Station* m, int feed,
+        int dim)
+{ return (feed == 0 && dim >= 0 && dim < 3) ?
+        m->element_true_enu_metres[dim] : 0; }
+const oskar_Mem* oskar_station_element_true_enu_metres_const(
+        const oskar_Station* m, int feed, int dim)
+{ return (feed == 0 && dim >= 0 && dim < 3) ?
+        m->element_true_enu_metres[dim] : 0; }
+oskar_Mem* oskar_station_element_measured_enu_metres(oskar_Station* m,
+        int feed, int dim)
+{ return (feed == 0 && dim >= 0 && dim < 3) ?
+        m->element_measured_enu_metres[dim] : 0; }
+const oskar_Mem* oskar_station_element_measured_enu_metres_const(
+        const oskar_Station* m, int feed, int dim)
+{ return (feed == 0 && dim >= 0 && dim < 3) ?
+        m->element_measured_enu_metres[dim] : 0; }
+oskar_Mem* oskar_station_element_cable_length_error_metres(oskar_Station* m,
+        int feed)
+{ return feed == 0 ? m->element_cable_length_error : 0; }
+const oskar_Mem* oskar_station_element_cable_length_error_metres_const(
+        const oskar_Station* m, int feed)
+{ return feed == 0 ? m->element_cable_length_error : 0; }
+oskar_Mem* oskar_station_element_types(oskar_Station* m)
+{ return m->element_types; }
+const oskar_Mem* oskar_station_element_types_const(const oskar_Station* m)
+{ return m->element_types; }
+double oskar_station_element_euler_index_rad(const oskar_Station* m,
+        int feed, int dim, int index)
+{
+    if (feed < 0 || feed > 1 || dim < 0 || dim > 2 || index < 0 ||
+            index >= m->num_elements) return 0.0;
+    return ((const double*) oskar_mem_void_const(
+            m->element_euler_cpu[feed][dim]))[index];
+}
+oskar_Element* oskar_station_element(oskar_Station* m, int i)
+{ return (i >= 0 && i < m->num_element_types) ? m->element[i] : 0; }
+const oskar_Element* oskar_station_element_const(const oskar_Station* m, int i)
+{ return (i >= 0 && i < m->num_element_types) ? m->element[i] : 0; }
+
+/* ---- Work buffers ------------------------------------------------------- */
+oskar_StationWork* oskar_station_work_create(int type, int location,
+        int* status)
+{
+    oskar_StationWork* w = 0;
+    if (*status) return 0;
+    if (type != OSKAR_SINGLE && type != OSKAR_DOUBLE)
+    {
+        *status = OSKAR_ERR_BAD_DATA_TYPE;
+        return 0;
+    }
+    w = (oskar_StationWork*) calloc(1, sizeof(oskar_StationWork));
+    if (!w) { *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE; return 0; }
+    w->precision = type;
+    w->mem_location = location;
+    w->weights = oskar_mem_create(type | OSKAR_COMPLEX, location, 0, status);
+    w->theta = oskar_mem_create(type, location, 0, status);
+    w->phi_x = oskar_mem_create(type, location, 0, status);
+    w->phi_y = oskar_mem_create(type, location, 0, status);
+    return w;
+}
+
+void oskar_station_work_free(oskar_StationWork* w, int* status)
+{
+    if (!w) return;
+    oskar_mem_free(w->weights, status);
+    oskar_mem_free(w->signal, status);
+    oskar_mem_free(w->theta, status);
+    oskar_mem_free(w->phi_x, status);
+    oskar_mem_free(w->phi_y, status);
+    free(w);
+}
+
+/* ---- Station-beam chain ------------------------------------------------- */
+void oskar_station_beam_horizon_direction(const oskar_Station* s,
+        const double gast_rad, double* x, double* y, double* z, int* status)
+{
+    if (*status) return;
+    if (s->beam_coord_type == OSKAR_COORDS_RADEC)
+    {
+        const double lst = gast_rad + s->lon_rad;
+        const double sin_lat = sin(s->lat_rad), cos_lat = cos(s->lat_rad);
+        const double ha = lst - s->beam_lon_rad;
+        const double sin_ha = sin(ha), cos_ha = cos(ha);
+        const double sin_dec = sin(s->beam_lat_rad);
+        const double cos_dec = cos(s->beam_lat_rad);
+        *x = -cos_dec * sin_ha;
+        *y = cos_lat * sin_dec - sin_lat * cos_dec * cos_ha;
+        *z = sin_lat * sin_dec + cos_lat * cos_dec * cos_ha;
+    }
+    else if (s->beam_coord_type == OSKAR_COORDS_AZEL)
+    {
+        const double cos_el = cos(s->beam_lat_rad);
+        *x = cos_el * sin(s->beam_lon_rad);
+        *y = cos_el * cos(s->beam_lon_rad);
+        *z = sin(s->beam_lat_rad);
+    }
+    else *status = OSKAR_ERR_INVALID_ARGUMENT;
+}
+
+static int check_gpu(const oskar_Mem* m, int* status)
+{
+    if (*status) return 0;
+    if (oskar_mem_location(m) != OSKAR_GPU)
+    {
+        *status = OSKAR_ERR_BAD_LOCATION;
+        return 0;
+    }
+    return 1;
+}
+
+void oskar_evaluate_element_weights_dft(int num_elements, const oskar_Mem* x,
+        const oskar_Mem* y, const oskar_Mem* z,
+        const oskar_Mem* cable_length_error, double wavenumber,
+        double x_beam, double y_beam, double z_beam, oskar_Mem* weights,
+        int* status)
+{
+    if (*status) return;
+    const int type = oskar_mem_type(x);
+    if (oskar_mem_type(y) != type || oskar_mem_type(z) != type ||
+            oskar_mem_type(cable_length_error) != type ||
+            oskar_mem_type(weights) != (type | OSKAR_COMPLEX))
+    {
+        *status = OSKAR_ERR_TYPE_MISMATCH;
+        return;
+    }
+    if (oskar_mem_location(y) != oskar_mem_location(x) ||
+            oskar_mem_location(z) != oskar_mem_location(x) ||
+            oskar_mem_location(cable_length_error) != oskar_mem_location(x) ||
+            oskar_mem_location(weights) != oskar_mem_location(x))
+    {
+        *status = OSKAR_ERR_LOCATION_MISMATCH;
+        return;
+    }
+    if ((int) oskar_mem_length(weights) < num_elements)
+    {
+        *status = OSKAR_ERR_DIMENSION_MISMATCH;
+        return;
+    }
+    if (!check_gpu(x, status)) return;
+    *status = oskar_amd_hip_status(oskar_amd_element_weights_dft_launch(
+            oskar_type_precision(type), num_elements, oskar_mem_void_const(x),
+            oskar_mem_void_const(y), oskar_mem_void_const(z),
+            oskar_mem_void_const(cable_length_error), wavenumber, x_beam,
+            y_beam, z_beam, oskar_mem_void(weights),
+            oskar_amd_current_stream()));
+}
+
+void oskar_convert_enu_directions_to_theta_phi(int offset_in, int num_points,
+        const oskar_Mem* x, const oskar_Mem* y, const oskar_Mem* z,
+        int extra_point_at_pole, double delta_phi1, double delta_phi2,
+        oskar_Mem* theta, oskar_Mem* phi1, oskar_Mem* phi2, int* status)
+{
+    if (*status) return;
+    const int type = oskar_mem_type(theta);
+    if (oskar_mem_type(x) != type || oskar_mem_type(y) != type ||
+            oskar_mem_type(z) != type || oskar_mem_type(phi1) != type ||
+            oskar_mem_type(phi2) != type)
+    {
+        *status = OSKAR_ERR_TYPE_MISMATCH;
+        return;
+    }
+    if (!check_gpu(x, status) || !check_gpu(theta, status)) return;
+    {
+        const size_t need = (size_t) num_points + (extra_point_at_pole ? 1 : 0);
+        if (oskar_mem_length(theta) < need || oskar_mem_length(phi1) < need ||
+                oskar_mem_length(phi2) < need ||
+                oskar_mem_length(x) < (size_t) (offset_in + num_points))
+        {
+            *status = OSKAR_ERR_DIMENSION_MISMATCH;
+            return;
+        }
+    }
+    *status = oskar_amd_hip_status(oskar_amd_theta_phi_launch(type, offset_in,
+            num_points, oskar_mem_void_const(x), oskar_mem_void_const(y),
+            oskar_mem_void_const(z), extra_point_at_pole, delta_phi1,
+            delta_phi2, oskar_mem_void(theta), oskar_mem_void(phi1),
+            oskar_mem_void(phi2), oskar_amd_current_stream()));
+}
+
+void oskar_evaluate_dipole_pattern(int num_points, const oskar_Mem* theta,
+        const oskar_Mem* phi_x, const oskar_Mem* phi_y, double freq_hz,
+        double dipole_length_m, int swap_xy, int offset_out,
+        oskar_Mem* pattern, int* status)
+{
+    if (*status) return;
+    const int precision = oskar_mem_precision(pattern);
+    const int location = oskar_mem_location(pattern);
+    const double kL = dipole_length_m * (M_PI * freq_hz / C_0);
+    const double cos_kL = cos(kL);
+    if (oskar_mem_location(theta) != location ||
+            oskar_mem_location(phi_x) != location ||
+            oskar_mem_location(phi_y) != location)
+    {
+        *status = OSKAR_ERR_LOCATION_MISMATCH;
+        return;
+    }
+    if (oskar_mem_type(theta) != precision ||
+            oskar_mem_type(phi_x) != precision ||
+            oskar_mem_type(phi_y) != precision)
+    {
+        *status = OSKAR_ERR_TYPE_MISMATCH;
+        return;
+    }
+    if (!oskar_mem_is_complex(pattern))
+    {
+        *status = OSKAR_ERR_BAD_DATA_TYPE;
+        return;
+    }
+    if (!check_gpu(pattern, status)) return;
+    if (oskar_mem_length(pattern) < (size_t) (offset_out + num_points) ||
+            (int) oskar_mem_length(theta) < num_points)
+    {
+        *status = OSKAR_ERR_DIMENSION_MISMATCH;
+        return;
+    }
+    *status = oskar_amd_hip_status(oskar_amd_dipole_launch(precision,
+            oskar_mem_is_matrix(pattern), num_points,
+            oskar_mem_void_const(theta), oskar_mem_void_const(phi_x),
+            oskar_mem_void_const(phi_y), kL, cos_kL, swap_xy, offset_out,
+            oskar_mem_void(pattern), oskar_amd_current_stream()));
+}
+
+void oskar_blank_below_horizon(int offset_mask, int num_sources,
+        const oskar_Mem* mask, int offset_out, oskar_Mem* jones, int* status)
+{
+    if (*status) return;
+    if (oskar_mem_precision(mask) != oskar_mem_precision(jones))
+    {
+        *status = OSKAR_ERR_TYPE_MISMATCH;
+        return;
+    }
+    if (oskar_mem_location(mask) != oskar_mem_location(jones))
+    {
+        *status = OSKAR_ERR_LOCATION_MISMATCH;
+        return;
+    }
+    if (!oskar_mem_is_complex(jones))
+    {
+        *status = OSKAR_ERR_BAD_DATA_TYPE;
+        return;
+    }
+    if (!check_gpu(jones, status)) return;
+    if (oskar_mem_length(mask) < (size_t) (offset_mask + num_sources) ||
+            oskar_mem_length(jones) < (size_t) (offset_out + num_sources))
+    {
+        *status = OSKAR_ERR_DIMENSION_MISMATCH;
+        return;
+    }
+    *status = oskar_amd_hip_status(oskar_amd_blank_below_horizon_launch(
+            oskar_mem_precision(jones), oskar_mem_is_matrix(jones),
+            offset_mask, num_sources, oskar_mem_void_const(mask), offset_out,
+            oskar_mem_void(jones), oskar_amd_current_stream()));
+}
+
+/* oskar_element_evaluate() for dipole and isotropic elements. */
+static void element_evaluate(const oskar_Element* model, int normalise,
+        int swap_xy, double orientation_x, double orientation_y,
+        int num_points, const oskar_Mem* x, const oskar_Mem* y,
+        const oskar_Mem* z, double frequency_hz, oskar_Mem* theta,
+        oskar_Mem* phi_x, oskar_Mem* phi_y, int offset_out, oskar_Mem* output,
+        int* status)
+{
+    double dipole_length_m = 0.0;
+    if (*status) return;
+    const int num_points_norm = normalise ? 1 + num_points : num_points;
+    oskar_mem_ensure(output, (size_t) (num_points_norm + offset_out), status);
+    oskar_mem_ensure(theta, (size_t) num_points_norm, status);
+    oskar_mem_ensure(phi_x, (size_t) num_points_norm, status);
+    oskar_mem_ensure(phi_y, (size_t) num_points_norm, status);
+    dipole_length_m = model->dipole_length;
+    if (model->dipole_length_units == OSKAR_WAVELENGTHS)
+        dipole_length_m *= (C_0 / frequency_hz);
+    oskar_convert_enu_directions_to_theta_phi(0, num_points, x, y, z,
+            normalise, (M_PI / 2) - orientation_x, (M_PI / 2) - orientation_y,
+            theta, phi_x, phi_y, status);
+    if (model->element_type == OSKAR_ELEMENT_TYPE_ISOTROPIC)
+        oskar_mem_set_value_real(output, 1.0, (size_t) offset_out,
+                (size_t) num_points_norm, status);
+    else if (model->element_type == OSKAR_ELEMENT_TYPE_DIPOLE)
+        oskar_evaluate_dipole_pattern(num_points_norm, theta, phi_x, phi_y,
+                frequency_hz, dipole_length_m, swap_xy, offset_out, output,
+                status);
+    else *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE;
+    if (normalise)
+        oskar_mem_normalise(output, (size_t) offset_out, (size_t) num_points,
+                (size_t) (offset_out + num_points), status);
+}
+
+void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
+        oskar_StationWork* work, int num_points, const oskar_Mem* x,
+        const oskar_Mem* y, const oskar_Mem* z, int time_index,
+        double gast_rad, double frequency_hz, oskar_Mem* beam, int* status)
+{
+    double beam_x = 0.0, beam_y = 0.0, beam_z = 0.0;
+    int i = 0;
+    (void) time_index;
+    if (*status) return;
+    if (s->station_type != OSKAR_STATION_TYPE_AA ||
+            !s->common_element_orientation || !s->common_pol_beams)
+    {
+        /* Per-element orientations, separate polarisation beams and other
+         * station kinds are not on this library's path. */
+        *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE;
+        return;
+    }
+    if (oskar_mem_location(beam) != OSKAR_GPU ||
+            s->mem_location != OSKAR_GPU || work->mem_location != OSKAR_GPU)
+    {
+        *status = OSKAR_ERR_BAD_LOCATION;
+        return;
+    }
+    if (!oskar_mem_is_complex(beam) ||
+            oskar_mem_precision(beam) != s->precision)
+    {
+        *status = OSKAR_ERR_TYPE_MISMATCH;
+        return;
+    }
+    const double wavenumber = 2.0 * M_PI * frequency_hz / C_0;
+    const int num_types = s->num_element_types;
+
+    oskar_station_beam_horizon_direction(s, gast_rad, &beam_x, &beam_y,
+            &beam_z, status);
+
+    /* Element patterns per element type: signal[type][num_points (+1)]. */
+    if (!work->signal || oskar_mem_type(work->signal) != oskar_mem_type(beam))
+    {
+        oskar_mem_free(work->signal, status);
+        work->signal = oskar_mem_create(oskar_mem_type(beam), OSKAR_GPU, 0,
+                status);
+    }
+    oskar_mem_ensure(work->signal,
+            (size_t) num_types * (size_t) (num_points + 1), status);
+    for (i = 0; i < num_types; ++i)
+    {
+        element_evaluate(s->element[i], s->normalise_element_pattern,
+                s->swap_xy,
+                oskar_station_element_euler_index_rad(s, 0, 0, 0) + M_PI / 2,
+                oskar_station_element_euler_index_rad(s, 1, 0, 0),
+                num_points, x, y, z, frequency_hz, work->theta, work->phi_x,
+                work->phi_y, i * num_points, work->signal, status);
+    }
+    oskar_mem_ensure(beam, (size_t) num_points, status);
+    if (s->enable_array_pattern)
+    {
+        oskar_mem_ensure(work->weights, (size_t) s->num_elements, status);
+        oskar_evaluate_element_weights_dft(s->num_elements,
+                s->element_measured_enu_metres[0],
+                s->element_measured_enu_metres[1],
+                s->element_measured_enu_metres[2],
+                s->element_cable_length_error, wavenumber, beam_x, beam_y,
+                beam_z, work->weights, status);
+        oskar_dftw(s->normalise_array_pattern, s->num_elements, wavenumber,
+                work->weights, s->element_true_enu_metres[0],
+                s->element_true_enu_metres[1], s->element_true_enu_metres[2],
+                0, num_points, x, y, s->array_is_3d ? z : 0, s->element_types,
+                work->signal, 1, 1, 0, beam, status);
+    }
+    else
+    {
+        if (num_types == 1)
+            oskar_mem_copy_contents(beam, work->signal, 0, 0,
+                    (size_t) num_points, status);
+        else
+        {
+            *status = OSKAR_ERR_SETTINGS_TELESCOPE;
+            return;
+        }
+    }
+    oskar_blank_below_horizon(0, num_points, z, 0, beam, status);
+}

@@ -1,0 +1,240 @@
+"""Python handles over the C-ABI structures (oskar_Mem, oskar_Jones,
+oskar_Telescope, oskar_Station, oskar_StationWork).
+
+Mirrors the way the reference's own tests drive the C API
+(oskar/correlate/test/Test_cross_correlate.cpp:60-120): create typed
+buffers in a location, fill them, call the hot-path function, read back.
+Data cross the boundary as numpy arrays copied through host aliases.
+"""
+import ctypes as C
+
+import numpy as np
+
+from .capi import lib
+
+OSKAR_CPU, OSKAR_GPU = 0, 1
+OSKAR_CHAR, OSKAR_INT, OSKAR_SINGLE, OSKAR_DOUBLE = 0x01, 0x02, 0x04, 0x08
+OSKAR_COMPLEX, OSKAR_MATRIX = 0x20, 0x40
+
+
+class OskarError(RuntimeError):
+    def __init__(self, status, where=""):
+        msg = lib.oskar_get_error_string(status)
+        msg = msg.decode() if msg else "unknown"
+        super().__init__("%s: status %d (%s)" % (where, status, msg))
+        self.status = status
+
+
+def check_status(status, where=""):
+    v = status.value if isinstance(status, C.c_int) else int(status)
+    if v != 0:
+        raise OskarError(v, where)
+
+
+def _np_dtype(type_code):
+    prec = type_code & 0x0F
+    if prec == OSKAR_INT:
+        return np.int32
+    if prec == OSKAR_CHAR:
+        return np.int8
+    return np.float64 if prec == OSKAR_DOUBLE else np.float32
+
+
+def _reals_per_element(type_code):
+    n = 1
+    if type_code & OSKAR_COMPLEX:
+        n *= 2
+    if type_code & OSKAR_MATRIX:
+        n *= 4
+    return n
+
+
+class Mem:
+    """Owning handle to an oskar_Mem."""
+
+    def __init__(self, type_code, location, num_elements, ptr=None):
+        self.type = type_code
+        self.location = location
+        if ptr is None:
+            st = C.c_int(0)
+            ptr = lib.oskar_mem_create(type_code, location, num_elements,
+                                       C.byref(st))
+            check_status(st, "oskar_mem_create")
+        self.ptr = ptr
+
+    @classmethod
+    def from_numpy(cls, arr, type_code, location=OSKAR_GPU):
+        """Copy a numpy array (reals, interleaved for complex) into a new
+        oskar_Mem of the given type and location."""
+        dt = _np_dtype(type_code)
+        a = np.ascontiguousarray(arr, dtype=dt).ravel()
+        per = _reals_per_element(type_code)
+        if a.size % per:
+            raise ValueError("array size not a multiple of element size")
+        n = a.size // per
+        m = cls(type_code, location, n)
+        if n:
+            st = C.c_int(0)
+            alias = lib.oskar_mem_create_alias_from_raw(
+                a.ctypes.data_as(C.c_void_p), type_code, OSKAR_CPU, n,
+                C.byref(st))
+            lib.oskar_mem_copy_contents(m.ptr, alias, 0, 0, n, C.byref(st))
+            lib.oskar_mem_free(alias, C.byref(C.c_int(0)))
+            check_status(st, "copy to oskar_Mem")
+        return m
+
+    def __len__(self):
+        return int(lib.oskar_mem_length(self.ptr))
+
+    def numpy(self):
+        """Copy contents to a new flat numpy array of reals."""
+        n = len(self)
+        out = np.zeros(n * _reals_per_element(self.type),
+                       dtype=_np_dtype(self.type))
+        if n:
+            st = C.c_int(0)
+            alias = lib.oskar_mem_create_alias_from_raw(
+                out.ctypes.data_as(C.c_void_p), self.type, OSKAR_CPU, n,
+                C.byref(st))
+            lib.oskar_mem_copy_contents(alias, self.ptr, 0, 0, n, C.byref(st))
+            lib.oskar_mem_free(alias, C.byref(C.c_int(0)))
+            check_status(st, "copy from oskar_Mem")
+        return out
+
+    def clear(self):
+        st = C.c_int(0)
+        lib.oskar_mem_clear_contents(self.ptr, C.byref(st))
+        check_status(st, "oskar_mem_clear_contents")
+
+    def data_ptr(self):
+        return lib.oskar_mem_void(self.ptr)
+
+    def free(self):
+        if self.ptr:
+            lib.oskar_mem_free(self.ptr, C.byref(C.c_int(0)))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class _Borrowed(Mem):
+    """Non-owning view of an oskar_Mem held by another structure."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+        self.type = lib.oskar_mem_type(ptr)
+        self.location = lib.oskar_mem_location(ptr)
+
+    def free(self):
+        self.ptr = None
+
+
+class Jones:
+    def __init__(self, type_code, location, num_stations, num_sources):
+        st = C.c_int(0)
+        self.ptr = lib.oskar_jones_create(type_code, location, num_stations,
+                                          num_sources, C.byref(st))
+        check_status(st, "oskar_jones_create")
+
+    @property
+    def mem(self):
+        return _Borrowed(lib.oskar_jones_mem(self.ptr))
+
+    def set(self, arr):
+        """Fill from a numpy array of reals ([station][source] order)."""
+        src = Mem.from_numpy(arr, lib.oskar_jones_type(self.ptr), OSKAR_CPU)
+        st = C.c_int(0)
+        lib.oskar_mem_copy_contents(lib.oskar_jones_mem(self.ptr), src.ptr,
+                                    0, 0, len(src), C.byref(st))
+        check_status(st, "Jones.set")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.oskar_jones_free(self.ptr, C.byref(C.c_int(0)))
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class Telescope:
+    def __init__(self, precision, location, num_stations):
+        st = C.c_int(0)
+        self.ptr = lib.oskar_telescope_create(precision, location,
+                                              num_stations, C.byref(st))
+        check_status(st, "oskar_telescope_create")
+
+    def set_station_ecef(self, x, y, z):
+        for dim, arr in enumerate((x, y, z)):
+            dst = _Borrowed(
+                lib.oskar_telescope_station_true_offset_ecef_metres(
+                    self.ptr, dim))
+            src = Mem.from_numpy(arr, dst.type, OSKAR_CPU)
+            st = C.c_int(0)
+            lib.oskar_mem_copy_contents(dst.ptr, src.ptr, 0, 0, len(src),
+                                        C.byref(st))
+            check_status(st, "set_station_ecef")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.oskar_telescope_free(self.ptr, C.byref(C.c_int(0)))
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class Station:
+    def __init__(self, precision, location, num_elements):
+        st = C.c_int(0)
+        self.ptr = lib.oskar_station_create(precision, location, num_elements,
+                                            C.byref(st))
+        check_status(st, "oskar_station_create")
+        self.precision = precision
+
+    def set_coords(self, true_xyz, measured_xyz=None):
+        """Bulk-set element ENU coordinates (3 x N arrays), then analyse."""
+        if measured_xyz is None:
+            measured_xyz = true_xyz
+        for dim in range(3):
+            for getter, arr in (
+                    (lib.oskar_station_element_true_enu_metres, true_xyz),
+                    (lib.oskar_station_element_measured_enu_metres,
+                     measured_xyz)):
+                dst = _Borrowed(getter(self.ptr, 0, dim))
+                src = Mem.from_numpy(arr[dim], self.precision, OSKAR_CPU)
+                st = C.c_int(0)
+                lib.oskar_mem_copy_contents(dst.ptr, src.ptr, 0, 0, len(src),
+                                            C.byref(st))
+                check_status(st, "Station.set_coords")
+        st = C.c_int(0)
+        lib.oskar_station_analyse(self.ptr, C.byref(C.c_int(0)), C.byref(st))
+        check_status(st, "oskar_station_analyse")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.oskar_station_free(self.ptr, C.byref(C.c_int(0)))
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class StationWork:
+    def __init__(self, precision, location):
+        st = C.c_int(0)
+        self.ptr = lib.oskar_station_work_create(precision, location,
+                                                 C.byref(st))
+        check_status(st, "oskar_station_work_create")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.oskar_station_work_free(self.ptr, C.byref(C.c_int(0)))
+                self.ptr = None
+        except Exception:
+            pass
