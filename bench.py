@@ -1,0 +1,361 @@
+"""Benchmark of the visibility hot path on one MI355X.
+
+Workload (one step = one time sample x one channel):
+  512 stations (log-radial layout to 35 km), 130,816 baselines;
+  1,000,000 point sources uniform over the sky above the horizon, Stokes I
+  from a power law (index -1.8, 0.01-10 Jy);
+  one 256-dipole aperture-array station design (random positions in a 38 m
+  disc) shared by all stations, beam pointed 30 deg from zenith;
+  150 MHz, no smearing, complex64 (FP32) visibilities.
+
+Measured:
+  value    source*baseline*channel evaluations per second of the factorised
+           pipeline: exact station beam (oskar_evaluate_station_beam_aperture_array,
+           FP32) + oskar_cross_correlate_factorised (split-FP16 MFMA);
+  plain    the same step through the plain FP32 pipeline of the reference:
+           station beam -> oskar_evaluate_jones_K -> oskar_jones_join ->
+           oskar_cross_correlate (direct FP32 kernel);
+  error    relative RMS of both against an FP64 / long-double CPU oracle on
+           a random sample of baselines at the full source count;
+  roofline the MFMA product kernel, timed with HIP events on its stream;
+  cpu      the reference's own CPU correlator (compiled from its sources into
+           oracle/_ref) on a bounded sample of the workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+C0 = 299792458.0
+METRIC = ("source·baseline·channel evals/s at 512 stations vs plain "
+          "FP32; rel RMS err vs FP64")
+PEAK_MFMA_TFLOPS = 2500.0   # MI355X dense FP16/BF16 MFMA
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def power_law(rng, n, index=-1.8, lo=0.01, hi=10.0):
+    """Inverse CDF of p(S) ~ S^index on [lo, hi]
+    (oskar/math/src/oskar_random_power_law.c:15-36)."""
+    u = rng.uniform(size=n)
+    a = index + 1.0
+    return (lo ** a + u * (hi ** a - lo ** a)) ** (1.0 / a)
+
+
+def make_workload(rng, num_stations, num_sources, num_elements):
+    r = np.exp(rng.uniform(np.log(100.0), np.log(35000.0), num_stations))
+    t = rng.uniform(0, 2 * np.pi, num_stations)
+    st = dict(u=r * np.cos(t), v=r * np.sin(t),
+              w=rng.uniform(-20.0, 20.0, num_stations))
+    z = rng.uniform(0.0, 1.0, num_sources)
+    z[z == 0.0] = 1e-7
+    ph = rng.uniform(0, 2 * np.pi, num_sources)
+    s = np.sqrt(1.0 - z * z)
+    sky = dict(l=s * np.sin(ph), m=s * np.cos(ph), n=z,
+               I=power_law(rng, num_sources))
+    re = 19.0 * np.sqrt(rng.uniform(size=num_elements))
+    te = rng.uniform(0, 2 * np.pi, num_elements)
+    el = np.stack([re * np.cos(te), re * np.sin(te), np.zeros(num_elements)])
+    return st, sky, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--stations", type=int, default=512)
+    ap.add_argument("--sources", type=int, default=1_000_000)
+    ap.add_argument("--elements", type=int, default=256)
+    ap.add_argument("--freq", type=float, default=150e6)
+    ap.add_argument("--mode", choices=["fp16x3", "bf16x3"], default="fp16x3")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--error-baselines", type=int, default=48)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X "
+              "(no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from oskar_amd import capi, lib
+    from oskar_amd.mem import (OSKAR_COMPLEX, OSKAR_GPU, OSKAR_MATRIX,
+                               OSKAR_SINGLE, Jones, Mem, Station,
+                               StationWork, Telescope, check_status)
+    assert lib.oskar_device_count_hip() > 0, "no HIP device visible"
+
+    N, S, f = args.stations, args.sources, args.freq
+    NB = N * (N - 1) // 2
+    evals = float(NB) * S
+    rng = np.random.default_rng(2024)
+    st, sky, el = make_workload(rng, N, S, args.elements)
+    log("workload: %d stations, %d sources, %d elements" % (N, S,
+                                                           args.elements))
+    F32 = OSKAR_SINGLE
+    CM = F32 | OSKAR_COMPLEX | OSKAR_MATRIX
+    mode = 0 if args.mode == "fp16x3" else 1
+
+    # Station model (shared design), beam 30 deg from zenith.
+    station = Station(F32, OSKAR_GPU, args.elements)
+    station.set_coords(el, el)
+    lib.oskar_station_set_phase_centre(station.ptr, 4, np.radians(0.0),
+                                       np.radians(60.0))
+    work = StationWork(F32, OSKAR_GPU)
+    dirs = [Mem.from_numpy(sky[k], F32) for k in ("l", "m", "n")]
+    flux = [Mem.from_numpy(sky["I"], F32)] + [
+        Mem.from_numpy(np.zeros(S), F32) for _ in range(3)]
+    uvw = [Mem.from_numpy(st[k], F32) for k in ("u", "v", "w")]
+    tel = Telescope(F32, OSKAR_GPU, N)
+    beam = Mem(CM, OSKAR_GPU, S)
+    vis = Mem(CM, OSKAR_GPU, NB)
+    status = C.c_int(0)
+
+    def step_beam():
+        lib.oskar_evaluate_station_beam_aperture_array(
+            station.ptr, work.ptr, S, dirs[0].ptr, dirs[1].ptr, dirs[2].ptr,
+            0, 0.0, f, beam.ptr, C.byref(status))
+
+    def step_fact():
+        step_beam()
+        lib.oskar_cross_correlate_factorised(
+            0, S, beam.ptr, capi.MEMA(*[m.ptr for m in flux]),
+            capi.MEM3(*[m.ptr for m in dirs]), tel.ptr,
+            capi.MEM3(*[m.ptr for m in uvw]), f, mode, 0, vis.ptr,
+            C.byref(status))
+
+    timer = lib.oskar_timer_create(1)
+
+    def timed(fn, steps, warmup, label):
+        for i in range(warmup):
+            fn()
+        lib.oskar_device_synchronize(OSKAR_GPU)
+        check_status(status, label)
+        log("%s: warmup done" % label)
+        lib.oskar_timer_start(timer)
+        for i in range(steps):
+            fn()
+        lib.oskar_device_synchronize(OSKAR_GPU)
+        t = lib.oskar_timer_elapsed(timer)
+        s = C.c_int(0)
+        lib.oskar_device_check_error_hip(C.byref(s))
+        check_status(status, label)
+        check_status(s, label)
+        log("%s: %d steps in %.3f s" % (label, steps, t))
+        return t
+
+    # --- factorised pipeline -------------------------------------------
+    vis.clear()
+    t_fact = timed(step_fact, args.steps, args.warmup, "factorised")
+    fact_rate = evals * args.steps / t_fact
+    ms_per_step = 1e3 * t_fact / args.steps
+
+    # Dominant kernel: MFMA product, HIP events on the launch stream.
+    lib.oskar_cross_correlate_factorised_set_profiling(1)
+    stage = np.zeros(3)
+    nprof = max(3, min(args.steps, 10))
+    for i in range(nprof):
+        step_fact()
+        ms = (C.c_double * 3)()
+        assert lib.oskar_cross_correlate_factorised_last_times(ms) == 0
+        stage += np.array(ms[:])
+    lib.oskar_cross_correlate_factorised_set_profiling(0)
+    stage /= nprof
+    # Beam time alone.
+    t_beam = timed(step_beam, args.steps, 1, "beam")
+    log("stage ms: prepare %.3f product %.3f reduce %.3f beam %.3f" % (
+        stage[0], stage[1], stage[2], 1e3 * t_beam / args.steps))
+    num_tiles = sum(min((N + 63) // 64,
+                        (min(128 * i + 127, N - 1) - 1) // 64 + 1)
+                    for i in range((N + 127) // 128))
+    part = 8192
+    s_pad = ((S + part - 1) // part) * part
+    exec_flops = num_tiles * float(s_pad) * 128 * 256 * 2 * 2 * 2 * 3
+    t_prod = stage[1] * 1e-3
+    achieved = 64.0 * evals / t_prod / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2),
+                "peak": PEAK_MFMA_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_MFMA_TFLOPS, 4),
+                "traffic": None,
+                "kernel": "fact_xcorr_kernel<_Float16>" if mode == 0 else
+                "fact_xcorr_kernel<__bf16>",
+                "kernel_ms": round(stage[1], 4),
+                "algorithmic_flops_per_eval": 64,
+                "mfma_executed_tflops": round(exec_flops / t_prod / 1e12, 2),
+                "mfma_executed_frac": round(exec_flops / t_prod / 1e12 /
+                                            PEAK_MFMA_TFLOPS, 4)}
+
+    # Keep one factorised result for the error check.
+    vis.clear()
+    step_fact()
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    vis_fact = vis.numpy()
+    beam_host = beam.numpy().astype(np.float64)
+
+    # --- plain FP32 pipeline ------------------------------------------
+    plain_rate = None
+    vis_plain = None
+    if not args.no_plain:
+        E = Jones(CM, OSKAR_GPU, N, S)
+        K = Jones(F32 | OSKAR_COMPLEX, OSKAR_GPU, N, S)
+        ext = [Mem(F32, OSKAR_GPU, 1) for _ in range(3)]
+        tel_p = Telescope(F32, OSKAR_GPU, N)
+        Emem = lib.oskar_jones_mem(E.ptr)
+
+        def step_plain():
+            step_beam()
+            for a in range(N):      # identical stations: duplicate the beam
+                lib.oskar_mem_copy_contents(Emem, beam.ptr, a * S, 0, S,
+                                            C.byref(status))
+            lib.oskar_evaluate_jones_K(K.ptr, S, dirs[0].ptr, dirs[1].ptr,
+                                       dirs[2].ptr, uvw[0].ptr, uvw[1].ptr,
+                                       uvw[2].ptr, f, flux[0].ptr, -1e30,
+                                       1e30, 0, C.byref(status))
+            lib.oskar_jones_join(E.ptr, K.ptr, E.ptr, C.byref(status))
+            lib.oskar_cross_correlate(
+                0, 0, S, E.ptr, capi.MEMA(*[m.ptr for m in flux]),
+                capi.MEM3(*[m.ptr for m in dirs]),
+                capi.MEM3(*[m.ptr for m in ext]), tel_p.ptr,
+                capi.MEM3(*[m.ptr for m in uvw]), 0.0, f, 0, vis.ptr,
+                C.byref(status))
+
+        vis.clear()
+        t_plain = timed(step_plain, args.plain_steps, 1, "plain")
+        plain_rate = evals * args.plain_steps / t_plain
+        vis.clear()
+        step_plain()
+        lib.oskar_device_synchronize(OSKAR_GPU)
+        vis_plain = vis.numpy()
+        del E, K
+
+    # --- error against FP64 ------------------------------------------
+    err_fact = err_plain = None
+    if not args.no_error:
+        import oracle
+        nb_s = min(args.error_baselines, NB)
+        pq = []
+        seen = set()
+        while len(pq) < nb_s:
+            p, q = sorted(rng.integers(0, N, 2))[::-1]
+            if p != q and (p, q) not in seen:
+                seen.add((p, q))
+                pq.append((p, q))
+        p = np.array([a for a, b in pq], np.int32)
+        q = np.array([b for a, b in pq], np.int32)
+        bl = q * (N - 1) - (q - 1) * q // 2 + p - q - 1
+        f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+        t0 = time.time()
+        ref = oracle.factorised_subset(
+            f32(sky["l"]), f32(sky["m"]), f32(sky["n"]), f32(sky["I"]),
+            beam_host, f32(st["u"]), f32(st["v"]), f32(st["w"]),
+            2 * np.pi * f / C0, p, q).reshape(-1, 8)
+        log("FP64 oracle on %d baselines x %d sources: %.1f s" % (
+            nb_s, S, time.time() - t0))
+
+        def rr(v):
+            got = v.reshape(-1, 8)[bl]
+            return float(np.sqrt(np.sum((got - ref) ** 2) / np.sum(ref ** 2)))
+        err_fact = rr(vis_fact)
+        if vis_plain is not None:
+            err_plain = rr(vis_plain)
+        log("rel RMS vs FP64: factorised %.3e plain %s" % (
+            err_fact, "%.3e" % err_plain if err_plain is not None else "-"))
+
+    # --- CPU baseline: the reference's own correlator ------------------
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+        ref_lib = oracle.ref()
+        kind = "reference" if ref_lib is not None else "port"
+
+        def cpu_run(ns):
+            # J = K E for the first ns sources, FP64, on the host.
+            k = 2 * np.pi * f / C0
+            ph = k * (np.outer(st["u"], sky["l"][:ns]) +
+                      np.outer(st["v"], sky["m"][:ns]) +
+                      np.outer(st["w"], sky["n"][:ns] - 1.0))
+            Kc = np.exp(1j * ph)
+            Ec = (beam_host[:8 * ns:2] + 1j * beam_host[1:8 * ns:2]).reshape(
+                ns, 4)
+            J = Kc[:, :, None] * Ec[None]
+            jr = np.ascontiguousarray(
+                np.stack([J.real, J.imag], -1).reshape(-1))
+            z = np.zeros(N)
+            zs = np.zeros(ns)
+            out = np.zeros(NB * 8)
+            arrs = [np.ascontiguousarray(a, np.float64) for a in (
+                sky["I"][:ns], zs, zs, zs, sky["l"][:ns], sky["m"][:ns],
+                sky["n"][:ns], st["u"], st["v"], st["w"], z, z)]
+            t0 = time.time()
+            if ref_lib is not None:
+                P = lambda a: a.ctypes.data_as(C.c_void_p)
+                D = C.c_double
+                ref_lib.oskar_cross_correlate_point_omp_d(
+                    C.c_int(0), C.c_int(ns), C.c_int(N), C.c_int(0), P(jr),
+                    *[P(a) for a in arrs], D(0.0), D(3.4e38), D(f / C0),
+                    D(0.0), D(0.0), D(0.0), D(0.0), P(out))
+            else:
+                oracle.cross_correlate("double", 0, 0, 1, jr, *arrs[:7],
+                                       None, None, None, *arrs[7:], freq=f)
+            return time.time() - t0
+
+        t1 = cpu_run(500)
+        ns = int(min(max(500, 500 * 4.0 / max(t1, 1e-3)), 50000))
+        times = [cpu_run(ns) for _ in range(3)]
+        tc = float(np.median(times))
+        cpu = {"value": NB * ns / tc, "unit": "evals/s", "cores": cores,
+               "kind": kind,
+               "sample": "%d stations x %d sources (of %d), FP64 matrix "
+                         "point-source correlator, median of 3" % (N, ns, S)}
+        log("CPU baseline (%s): %.3e evals/s on %d cores" % (
+            kind, cpu["value"], cores))
+
+    line = {
+        "metric": METRIC,
+        "value": fact_rate,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16x3" if mode == 0 else "bf16x3",
+        "data": "synthetic",
+        "config": {
+            "workload": "512 stations x 1M point sources, 1 channel, "
+                        "shared 256-dipole station beam (exact FP32 DFT), "
+                        "no smearing (factorised MFMA pipeline)",
+            "stations": N, "baselines": NB, "sources": S,
+            "elements": args.elements, "freq_hz": f, "mode": args.mode,
+            "parallelism": "single GPU"},
+        "plain_fp32_evals_per_s": plain_rate,
+        "speedup_vs_plain_fp32": (fact_rate / plain_rate) if plain_rate
+        else None,
+        "rel_rms_vs_fp64": err_fact,
+        "rel_rms_plain_fp32_vs_fp64": err_plain,
+        "stage_ms": {"prepare": round(stage[0], 4),
+                     "product": round(stage[1], 4),
+                     "reduce": round(stage[2], 4),
+                     "beam": round(1e3 * t_beam / args.steps, 4)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

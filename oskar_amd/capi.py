@@ -101,6 +101,8 @@ SIGNATURES = {
     "oskar_jones_mem": (P, [P]),
     "oskar_jones_mem_const": (P, [P]),
     "oskar_jones_join": (None, [P, P, P, IP]),
+    "oskar_evaluate_jones_K": (None, [P, I, P, P, P, P, P, P, D, P, D, D, I,
+                                      IP]),
     # oskar_telescope.h
     "oskar_telescope_create": (P, [I, I, I, IP]),
     "oskar_telescope_free": (None, [P, IP]),
@@ -133,6 +135,12 @@ SIGNATURES = {
     "oskar_cross_correlate_scalar_point_hip_d": (None, _XS_D),
     "oskar_cross_correlate_scalar_gaussian_hip_f": (None, _XSG_F),
     "oskar_cross_correlate_scalar_gaussian_hip_d": (None, _XSG_D),
+    # oskar_cross_correlate_factorised.h
+    "oskar_cross_correlate_factorised": (None, [I, I, P, MEMA, MEM3, P, MEM3,
+                                                D, I, I, P, IP]),
+    "oskar_cross_correlate_factorised_set_part_size": (I, [I]),
+    "oskar_cross_correlate_factorised_set_profiling": (None, [I]),
+    "oskar_cross_correlate_factorised_last_times": (I, [DP]),
     # oskar_dftw.h
     "oskar_dftw": (None, [I, I, D, P, P, P, P, I, I, P, P, P, P, P, I, I, I,
                           P, IP]),

@@ -455,3 +455,57 @@ extern "C" int oskar_amd_jones_join_launch(int precision, int a_matrix,
 #undef JOIN
     return (int) hipGetLastError();
 }
+
+namespace {
+
+/* K[a][s] (define_evaluate_jones_K.h:56-75, CPU evaluation order). */
+template <typename FP>
+__global__ __launch_bounds__(256)
+void jones_K_kernel(int num_sources, const FP* l, const FP* m, const FP* n,
+        int num_stations, const FP* u, const FP* v, const FP* w,
+        FP wavenumber, const FP* filter, FP fmin, FP fmax, int ignore_w,
+        Cplx<FP>* jones)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (s >= num_sources || a >= num_stations) return;
+    Cplx<FP> k;
+    k.x = k.y = (FP) 0;
+    const FP f = filter[s];
+    if (f > fmin && f <= fmax)
+    {
+        FP phase = u[a] * l[s] + v[a] * m[s];
+        if (!ignore_w) phase += w[a] * (n[s] - (FP) 1);
+        phase *= wavenumber;
+        sincos_v<FP>(phase, &k.y, &k.x);
+    }
+    jones[(size_t) s + (size_t) num_sources * a] = k;
+}
+
+} /* namespace */
+
+extern "C" int oskar_amd_jones_K_launch(int precision, int num_sources,
+        const void* l, const void* m, const void* n, int num_stations,
+        const void* u, const void* v, const void* w, double wavenumber,
+        const void* filter, double fmin, double fmax, int ignore_w,
+        void* jones, void* stream)
+{
+    if (num_sources <= 0 || num_stations <= 0) return 0;
+    const dim3 grid((num_sources + 255) / 256, num_stations);
+    const hipStream_t s = (hipStream_t) stream;
+    if (precision == OSKAR_DOUBLE)
+        hipLaunchKernelGGL(jones_K_kernel<double>, grid, dim3(256), 0, s,
+                num_sources, (const double*) l, (const double*) m,
+                (const double*) n, num_stations, (const double*) u,
+                (const double*) v, (const double*) w, wavenumber,
+                (const double*) filter, fmin, fmax, ignore_w,
+                (Cplx<double>*) jones);
+    else
+        hipLaunchKernelGGL(jones_K_kernel<float>, grid, dim3(256), 0, s,
+                num_sources, (const float*) l, (const float*) m,
+                (const float*) n, num_stations, (const float*) u,
+                (const float*) v, (const float*) w, (float) wavenumber,
+                (const float*) filter, (float) fmin, (float) fmax, ignore_w,
+                (Cplx<float>*) jones);
+    return (int) hipGetLastError();
+}

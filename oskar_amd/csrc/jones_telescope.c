@@ -9,6 +9,11 @@
 #include <stdlib.h>
 
 #include "private.h"
+#include "oskar_evaluate_jones_K.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 
 /* ---- Jones ---------------------------------------------------------------- */
 oskar_Jones* oskar_jones_create(int type, int location, int num_stations,
@@ -123,6 +128,76 @@ void oskar_jones_join(oskar_Jones* j3, oskar_Jones* j1, const oskar_Jones* j2,
                 j3->data->data, j1->data->data, j2->data->data,
                 oskar_amd_current_stream()));
     }
+}
+
+/* oskar/interferometer/src/oskar_evaluate_jones_K.c:19-157 */
+void oskar_evaluate_jones_K(oskar_Jones* K, int num_sources,
+        const oskar_Mem* l, const oskar_Mem* m, const oskar_Mem* n,
+        const oskar_Mem* u, const oskar_Mem* v, const oskar_Mem* w,
+        double frequency_hz, const oskar_Mem* source_filter,
+        double source_filter_min, double source_filter_max,
+        int ignore_w_components, int* status)
+{
+    if (*status) return;
+    const int type = oskar_jones_type(K);
+    const int precision = oskar_type_precision(type);
+    const int location = oskar_jones_mem_location(K);
+    const int num_stations = oskar_jones_num_stations(K);
+    const double wavenumber = 2.0 * M_PI * frequency_hz / 299792458.0;
+    if (oskar_mem_location(l) != location ||
+            oskar_mem_location(m) != location ||
+            oskar_mem_location(n) != location ||
+            oskar_mem_location(source_filter) != location ||
+            oskar_mem_location(u) != location ||
+            oskar_mem_location(v) != location ||
+            oskar_mem_location(w) != location)
+    {
+        *status = OSKAR_ERR_LOCATION_MISMATCH;
+        return;
+    }
+    if (precision != oskar_mem_type(l) || precision != oskar_mem_type(m) ||
+            precision != oskar_mem_type(n) || precision != oskar_mem_type(u) ||
+            precision != oskar_mem_type(v) || precision != oskar_mem_type(w) ||
+            precision != oskar_mem_type(source_filter))
+    {
+        *status = OSKAR_ERR_TYPE_MISMATCH;
+        return;
+    }
+    if (type != OSKAR_SINGLE_COMPLEX && type != OSKAR_DOUBLE_COMPLEX)
+    {
+        *status = OSKAR_ERR_BAD_DATA_TYPE;
+        return;
+    }
+    if (oskar_jones_num_sources(K) < num_sources ||
+            (int) oskar_mem_length(l) < num_sources ||
+            (int) oskar_mem_length(m) < num_sources ||
+            (int) oskar_mem_length(n) < num_sources ||
+            (int) oskar_mem_length(source_filter) < num_sources ||
+            (int) oskar_mem_length(u) < num_stations ||
+            (int) oskar_mem_length(v) < num_stations ||
+            (int) oskar_mem_length(w) < num_stations)
+    {
+        *status = OSKAR_ERR_DIMENSION_MISMATCH;
+        return;
+    }
+    if (location != OSKAR_GPU)
+    {
+        *status = OSKAR_ERR_BAD_LOCATION;
+        return;
+    }
+    /* The kernel writes with the Jones source stride. */
+    if (oskar_jones_num_sources(K) != num_sources)
+    {
+        *status = OSKAR_ERR_DIMENSION_MISMATCH;
+        return;
+    }
+    *status = oskar_amd_hip_status(oskar_amd_jones_K_launch(precision,
+            num_sources, oskar_mem_void_const(l), oskar_mem_void_const(m),
+            oskar_mem_void_const(n), num_stations, oskar_mem_void_const(u),
+            oskar_mem_void_const(v), oskar_mem_void_const(w), wavenumber,
+            oskar_mem_void_const(source_filter), source_filter_min,
+            source_filter_max, ignore_w_components,
+            oskar_mem_void(K->data), oskar_amd_current_stream()));
 }
 
 /* ---- Telescope ------------------------------------------------------------ */

@@ -178,6 +178,11 @@ int oskar_amd_mem_normalise_launch(int type, unsigned int offset,
         unsigned int n, void* data, unsigned int idx, void* stream);
 int oskar_amd_mem_set_value_real_launch(int type, size_t offset, size_t n,
         double value, void* data, void* stream);
+int oskar_amd_jones_K_launch(int precision, int num_sources,
+        const void* l, const void* m, const void* n, int num_stations,
+        const void* u, const void* v, const void* w, double wavenumber,
+        const void* filter, double fmin, double fmax, int ignore_w,
+        void* jones, void* stream);
 int oskar_amd_jones_join_launch(int precision, int a_matrix, int b_matrix,
         size_t n, void* out, const void* a, const void* b, void* stream);
 

@@ -94,3 +94,36 @@ def gpu_dftw(prec, matrix, normalise, wavenumber, weights, x_in, y_in, z_in,
     check_status(s, "oskar_dftw")
     sync_check("oskar_dftw")
     return out.numpy()
+
+
+def gpu_factorised(prec, mode, casa, beam, I_, Q, U, V, l, m, n, u, v, w,
+                   freq, vis_init=None, offset_out=0, uv_filter=None,
+                   num_sources=None):
+    """oskar_cross_correlate_factorised on the GPU; returns vis reals."""
+    P = prec_code(prec)
+    ns = u.size
+    nsrc = l.size if num_sources is None else num_sources
+    mt = P | OSKAR_COMPLEX | OSKAR_MATRIX
+    tel = Telescope(P, OSKAR_GPU, ns)
+    if uv_filter is not None:
+        s = C.c_int(0)
+        lib.oskar_telescope_set_uv_filter(tel.ptr, uv_filter[0], uv_filter[1],
+                                          b"Metres", C.byref(s))
+        check_status(s, "set_uv_filter")
+    flux = [Mem.from_numpy(a, P) for a in (I_, Q, U, V)]
+    dirs = [Mem.from_numpy(a, P) for a in (l, m, n)]
+    uvw = [Mem.from_numpy(a, P) for a in (u, v, w)]
+    mb = Mem.from_numpy(beam, mt) if beam is not None else None
+    nb = ns * (ns - 1) // 2
+    if vis_init is None:
+        vis_init = np.zeros((offset_out + nb) * 8)
+    vis = Mem.from_numpy(vis_init, mt)
+    s = C.c_int(0)
+    lib.oskar_cross_correlate_factorised(
+        int(casa), int(nsrc), mb.ptr if mb else None,
+        capi.MEMA(*[x.ptr for x in flux]), capi.MEM3(*[x.ptr for x in dirs]),
+        tel.ptr, capi.MEM3(*[x.ptr for x in uvw]), float(freq), int(mode),
+        int(offset_out), vis.ptr, C.byref(s))
+    check_status(s, "oskar_cross_correlate_factorised")
+    sync_check("oskar_cross_correlate_factorised")
+    return vis.numpy()
