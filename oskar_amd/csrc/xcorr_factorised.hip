@@ -21,8 +21,10 @@
  *  - P and W are split into hi + lo 16-bit parts (FP16 or BF16); each
  *    product is formed as hi*hi + hi*lo + lo*hi on the MFMA units with FP32
  *    accumulation ("x3");
- *  - C is scaled by a global power of two (from a device-side maximum) so
- *    the FP16 parts stay in range; the scale is removed exactly at the end;
+ *  - K is scaled by 2^12 and C by a global power of two (from a device-side
+ *    maximum) so the FP16 parts stay in range and the lo parts of small
+ *    components stay normal (the matrix cores do not keep FP16 subnormal
+ *    inputs); both scales are removed exactly at the end;
  *  - each workgroup accumulates at most SRC_PER_PART sources in FP32, and
  *    the per-part partial sums are added in FP64 by the reduction kernel
  *    (cross-tile sums in FP64), in a fixed order (no atomics on the data).
@@ -37,7 +39,8 @@
  * 4 P values and 2 x 4 W values into LDS (double-buffered, 64 KB), then each
  * wave issues 48 v_mfma_f32_32x32x16 instructions.  LDS operand planes are
  * laid out [k-half][row][8] so a wave's 16-byte fragment reads are
- * contiguous.
+ * contiguous; W rows are ordered (ik, q) so the builders' writes are too.
+ * Source data for step k+2 is loaded into registers during step k.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -156,6 +159,10 @@ struct FactArgs
     float* partial;              /* [part][tile][re/im][TP][TW] */
 };
 
+/* Scale of the K operand: |K| <= 1 -> <= 2^12, so lo parts of components
+ * down to 2^-15 stay normal FP16 numbers. */
+constexpr float K_SCALE = 4096.0f;
+
 /* Power-of-two scale that puts max |C| just below 2^14. */
 __device__ __forceinline__ float c_scale(const unsigned int* cmax)
 {
@@ -164,6 +171,31 @@ __device__ __forceinline__ float c_scale(const unsigned int* cmax)
     int e = 0;
     (void) frexpf(m, &e);          /* m = f * 2^e, 0.5 <= f < 1 */
     return ldexpf(1.0f, 14 - e);
+}
+
+/* Source data one thread needs for one k-step: 4 sources for its P row,
+ * 2 sources (with C) for its W rows.  Loaded two k-steps ahead so the
+ * global-memory latency hides behind the MFMAs of the current step. */
+struct StepSrc
+{
+    Src p[4];
+    Src w[2];
+    float4 c[2];
+};
+
+__device__ __forceinline__ void load_step(const FactArgs& A, int sb,
+        int s_last, int pg, int qg, StepSrc* o)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        o->p[j] = A.src[min(sb + pg * 4 + j, s_last)];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+    {
+        const int s = min(sb + qg * 2 + j, s_last);
+        o->w[j] = A.src[s];
+        o->c[j] = A.cmat[s];
+    }
 }
 
 template <typename T16>
@@ -181,44 +213,38 @@ void fact_xcorr_kernel(FactArgs A)
     const int lane = tid & 63, wave = tid >> 6;
     const float scale = c_scale(A.cmax);
 
-    /* Station constants for this thread's P and W builds (zero outside). */
+    /* Station constants for this thread's P and W builds; stations past N
+     * get a zero mask (their rows build as zeros). */
     const int pb = tid & (TP - 1), pg = tid >> 7;     /* P: station, half */
     const int qb = tid & (TQ - 1), qg = tid >> 6;     /* W: station, group */
-    double pu = 0.0, pv = 0.0, pw = 0.0, qu = 0.0, qv = 0.0, qw = 0.0;
-    const bool p_ok = p0 + pb < N, q_ok = q0 + qb < N;
-    if (p_ok)
-    {
-        pu = A.st_uvw[3 * (p0 + pb)];
-        pv = A.st_uvw[3 * (p0 + pb) + 1];
-        pw = A.st_uvw[3 * (p0 + pb) + 2];
-    }
-    if (q_ok)
-    {
-        qu = A.st_uvw[3 * (q0 + qb)];
-        qv = A.st_uvw[3 * (q0 + qb) + 1];
-        qw = A.st_uvw[3 * (q0 + qb) + 2];
-    }
+    const int ps = min(p0 + pb, N - 1), qs = min(q0 + qb, N - 1);
+    const double pu = A.st_uvw[3 * ps], pv = A.st_uvw[3 * ps + 1],
+            pw = A.st_uvw[3 * ps + 2];
+    const double qu = A.st_uvw[3 * qs], qv = A.st_uvw[3 * qs + 1],
+            qw = A.st_uvw[3 * qs + 2];
+    const float p_mask = (p0 + pb < N) ? K_SCALE : 0.0f;
+    const float q_mask = (q0 + qb < N) ? scale : 0.0f;
 
     const int s_begin = part * A.src_per_part;
     const int s_end = min(A.num_sources, s_begin + A.src_per_part);
+    const int s_last = s_end - 1;
     const int nsteps = (s_end - s_begin + KSRC - 1) / KSRC;
 
-    /* Builds the P and W operand planes of k-step ks into stage buffer. */
-    auto build = [&](int ks, uint4* st) {
-        const int sb = s_begin + ks * KSRC;
-        /* P: station pb, sources pg*4 .. pg*4+3 -> one k-half. */
+    /* Builds the P and W operand planes of the k-step starting at source sb
+     * from the prefetched source data, into one stage buffer.  Branch-free:
+     * sources past the end of this part are masked to zero. */
+    auto build = [&](int sb, const StepSrc& d, uint4* st) {
         {
             T16 xh[8], xl[8], yh[8], yl[8];
 #pragma unroll
             for (int j = 0; j < 4; ++j)
             {
-                const int s = sb + pg * 4 + j;
-                float re = 0.0f, im = 0.0f;
-                if (p_ok && s < s_end)
-                {
-                    const Src q = A.src[s];
-                    phasor(pu * q.l + pv * q.m + pw * q.n1, &re, &im);
-                }
+                const float mk = (sb + pg * 4 + j <= s_last) ? p_mask : 0.0f;
+                float re, im;
+                phasor(pu * d.p[j].l + pv * d.p[j].m + pw * d.p[j].n1, &re,
+                        &im);
+                re *= mk;
+                im *= mk;
                 Split16<T16>::split(re, &xh[2 * j], &xl[2 * j]);
                 Split16<T16>::split(im, &xh[2 * j + 1], &xl[2 * j + 1]);
                 /* Y = (Im, -Re): negation is exact in the split parts. */
@@ -233,23 +259,19 @@ void fact_xcorr_kernel(FactArgs A)
             __builtin_memcpy(&v, yh, 16); st[(2 * 2 + pg) * TP + pb] = v;
             __builtin_memcpy(&v, yl, 16); st[(3 * 2 + pg) * TP + pb] = v;
         }
-        /* W: station qb, sources qg*2, qg*2+1 -> k-half qg/2, 4 reals at
-         * offset (qg&1)*4 of each of the 4 rows (qb*4 + ik). */
         {
+            /* W rows are ordered ik-major (row = ik * TQ + q), so the 64
+             * threads of a wave write 64 consecutive rows. */
             T16 wh[4][4], wl[4][4];
 #pragma unroll
             for (int j = 0; j < 2; ++j)
             {
-                const int s = sb + qg * 2 + j;
-                float kr = 0.0f, ki = 0.0f;
-                float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (q_ok && s < s_end)
-                {
-                    const Src q = A.src[s];
-                    phasor(qu * q.l + qv * q.m + qw * q.n1, &kr, &ki);
-                    c = A.cmat[s];
-                    c.x *= scale; c.y *= scale; c.z *= scale; c.w *= scale;
-                }
+                const float mk = (sb + qg * 2 + j <= s_last) ? q_mask : 0.0f;
+                float kr, ki;
+                phasor(qu * d.w[j].l + qv * d.w[j].m + qw * d.w[j].n1, &kr,
+                        &ki);
+                const float4 c = make_float4(d.c[j].x * mk, d.c[j].y * mk,
+                        d.c[j].z * mk, d.c[j].w * mk);
                 /* W_ik = K conj(C_ik): C00, C11 real; C10 = conj(C01). */
                 float w[4][2];
                 w[0][0] = kr * c.x;  w[0][1] = ki * c.x;
@@ -269,7 +291,7 @@ void fact_xcorr_kernel(FactArgs A)
 #pragma unroll
             for (int ik = 0; ik < 4; ++ik)
             {
-                const int row = qb * 4 + ik;
+                const int row = ik * TQ + qb;
                 uint2 v;
                 __builtin_memcpy(&v, wh[ik], 8);
                 reinterpret_cast<uint2*>(
@@ -291,12 +313,20 @@ void fact_xcorr_kernel(FactArgs A)
 
     const int wp = wave & 1, ww = wave >> 1;
     const int lr = lane & 31, lh = lane >> 5;
-    if (nsteps > 0) build(0, lds[0]);
+    StepSrc cur;
+    if (nsteps > 0)
+    {
+        load_step(A, s_begin, s_last, pg, qg, &cur);
+        build(s_begin, cur, lds[0]);
+        load_step(A, s_begin + KSRC, s_last, pg, qg, &cur);
+    }
     __syncthreads();
     for (int ks = 0; ks < nsteps; ++ks)
     {
         const uint4* st = lds[ks & 1];
-        if (ks + 1 < nsteps) build(ks + 1, lds[(ks + 1) & 1]);
+        /* Operand fragments of step ks first: the builder's LDS writes to
+         * the other stage come after them in program order, so the MFMAs
+         * need not wait for the build. */
         uint4 xh[2], xl[2], yh[2], yl[2], wh[4], wl[4];
 #pragma unroll
         for (int rf = 0; rf < 2; ++rf)
@@ -340,11 +370,28 @@ void fact_xcorr_kernel(FactArgs A)
                 re[rf][cf] = mfma<T16>(xl[rf], wh[cf], re[rf][cf]);
                 im[rf][cf] = mfma<T16>(yl[rf], wh[cf], im[rf][cf]);
             }
+        /* Build step ks+1 (a masked dummy after the last step) into the
+         * other stage, then start the loads for step ks+2. */
+        build(s_begin + (ks + 1) * KSRC, cur, lds[(ks + 1) & 1]);
+        load_step(A, s_begin + (ks + 2) * KSRC, s_last, pg, qg, &cur);
+        /* Schedule: operand reads, then each MFMA followed by a few VALU
+         * instructions of the build, then the builder's LDS writes and the
+         * next loads.  The matrix pipe runs while the VALU builds. */
+        __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);
+#pragma unroll
+        for (int i = 0; i < 48; ++i)
+        {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x200, 8, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 14, 0);
         __syncthreads();
     }
 
     /* Store this part's FP32 tile: element (row i, col j) of a 32x32 block
-     * sits in lane (j + 32 * ((i / 4) % 2)), register 4 * (i / 8) + i % 4. */
+     * sits in lane (j + 32 * ((i / 4) % 2)), register 4 * (i / 8) + i % 4.
+     * Columns are W rows, ordered ik * TQ + q. */
     float* base = A.partial + ((size_t) part * A.num_tiles + tile) *
             (size_t) (2 * TP * TW);
 #pragma unroll
@@ -446,17 +493,17 @@ __global__ void fact_stations_kernel(int n, const FP* u, const FP* v,
 }
 
 /* Sum of the per-part tiles in FP64, scale removal, and accumulation into
- * vis[offset_out + b] for p > q outside the uv filter.  Thread per
- * (q, p) pair: grid x over p, y = q. */
+ * vis[offset_out + b] for p > q outside the uv filter.  Thread per (p, q):
+ * grid y = p, x over q, so consecutive threads read consecutive partials. */
 template <typename FP>
 __global__ __launch_bounds__(256)
 void fact_reduce_kernel(int N, int num_tiles, int num_parts, int use_casa,
         const float* partial, const unsigned int* cmax, const double* st_uvw,
         double uv_min, double uv_max, int offset_out, Cplx<FP>* vis)
 {
-    const int q = blockIdx.y;
-    const int p = q + 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= N) return;
+    const int p = blockIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= p) return;
     /* uv filter on the baseline length in wavelengths. */
     {
         const double du = st_uvw[3 * p] - st_uvw[3 * q];
@@ -471,15 +518,15 @@ void fact_reduce_kernel(int N, int num_tiles, int num_parts, int use_casa,
     for (int k = 0; k < num_parts; ++k)
     {
         const float* b = partial + ((size_t) k * num_tiles + t) *
-                (size_t) (2 * TP * TW) + (size_t) pl * TW + ql * 4;
+                (size_t) (2 * TP * TW) + (size_t) pl * TW + ql;
 #pragma unroll
         for (int ik = 0; ik < 4; ++ik)
         {
-            vr[ik] += (double) b[ik];
-            vi[ik] += (double) b[TP * TW + ik];
+            vr[ik] += (double) b[ik * TQ];
+            vi[ik] += (double) b[TP * TW + ik * TQ];
         }
     }
-    const double inv = 1.0 / (double) c_scale(cmax);
+    const double inv = 1.0 / ((double) c_scale(cmax) * (double) K_SCALE);
     const int bl = q * (N - 1) - (q - 1) * q / 2 + p - q - 1;
     Cplx<FP>* o = vis + 4 * (size_t) (offset_out + bl);
 #pragma unroll
@@ -638,12 +685,12 @@ extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
     mark(dev, 2, stream);
     if (precision == OSKAR_DOUBLE)
         hipLaunchKernelGGL(fact_reduce_kernel<double>,
-                dim3((N + 255) / 256, N - 1), dim3(256), 0, stream, N, tiles,
+                dim3((N + 255) / 256, N), dim3(256), 0, stream, N, tiles,
                 parts, use_casa, partial, cmax, st, uv_min, uv_max,
                 offset_out, (Cplx<double>*) vis);
     else
         hipLaunchKernelGGL(fact_reduce_kernel<float>,
-                dim3((N + 255) / 256, N - 1), dim3(256), 0, stream, N, tiles,
+                dim3((N + 255) / 256, N), dim3(256), 0, stream, N, tiles,
                 parts, use_casa, partial, cmax, st, uv_min, uv_max,
                 offset_out, (Cplx<float>*) vis);
     mark(dev, 3, stream);

@@ -3,8 +3,11 @@ CPU oracle of the reference's direct algorithm, through the C-ABI.
 
 The oracle forms J_p(s) = K_p(s) E(s) (oskar_evaluate_jones_K semantics)
 and runs the reference's double-precision cross-correlation on it.
-Tolerances (relative RMS of all visibility reals vs FP64):
-  FP16x3: 1e-5 (expected about 1e-6);  BF16x3: 5e-5 (about 1e-5).
+Tolerances (relative RMS of all visibility reals vs FP64 on the same
+rounded inputs):  FP16x3 and BF16x3: 1.5e-5.  The FP16 split alone would
+allow ~1e-7, but the matrix core sums the 16 products of one instruction in
+a single aligned adder, so low bits of faint sources that share a k-step
+with bright ones are truncated; measured 1e-7 .. 9e-6 on these skies.
 """
 import numpy as np
 import pytest
@@ -15,7 +18,7 @@ from helpers import max_rel, rel_rms
 
 pytestmark = pytest.mark.gpu
 C0 = 299792458.0
-TOL = {0: 1e-5, 1: 5e-5}
+TOL = {0: 1.5e-5, 1: 1.5e-5}
 
 
 def make_case(seed, ns, nsrc, spread=20000.0, pol=True, beam=True):
@@ -41,6 +44,20 @@ def make_case(seed, ns, nsrc, spread=20000.0, pol=True, beam=True):
         E[:, 0] += 1.0
         E[:, 6] += 1.0
     return dict(u=u, v=v, w=w, l=l, m=m, n=n, I=I_, Q=Q, U=U, V=V, E=E)
+
+
+def rounded(c, prec):
+    """The inputs as the GPU receives them: FP32 runs see FP32-rounded
+    positions and directions, whose rounding alone moves phases by ~1e-3 rad
+    on 10 km baselines, so each precision is checked against the oracle on
+    the same rounded inputs."""
+    if prec == "d":
+        return c
+    r = {}
+    for k, v in c.items():
+        r[k] = None if v is None else np.asarray(v, np.float32).astype(
+            np.float64)
+    return r
 
 
 def oracle_direct(c, freq, casa=0, **kw):
@@ -77,8 +94,8 @@ def test_factorised_matches_fp64_oracle(hip, mode, shape):
     print("factorised", mode, shape, flush=True)
     c = make_case(ns + nsrc, ns, nsrc)
     freq = 150e6
-    want = oracle_direct(c, freq)
     for prec in ("f", "d"):
+        want = oracle_direct(rounded(c, prec), freq)
         got = run(c, prec, mode, freq)
         err = rel_rms(got, want)
         print("  prec %s rel RMS %.3e  max rel %.3e" % (
@@ -107,7 +124,7 @@ def test_factorised_part_size_does_not_change_result_beyond_tolerance(hip):
     from oskar_amd import lib
     c = make_case(4, 96, 12000)
     freq = 200e6
-    want = oracle_direct(c, freq)
+    want = oracle_direct(rounded(c, "f"), freq)
     old = lib.oskar_cross_correlate_factorised_set_part_size(1024)
     try:
         a = run(c, "f", 0, freq)
