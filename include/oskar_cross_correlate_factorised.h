@@ -70,6 +70,80 @@ OSKAR_EXPORT int oskar_cross_correlate_factorised_set_part_size(int sources);
 OSKAR_EXPORT void oskar_cross_correlate_factorised_set_profiling(int on);
 OSKAR_EXPORT int oskar_cross_correlate_factorised_last_times(double ms[3]);
 
+/*
+ * Batches of (channel, time) units in one persistent launch.
+ *
+ * No counterpart in the reference: OSKAR calls the correlator once per
+ * (time, channel, source chunk) from sim_baselines
+ * (oskar/interferometer/src/oskar_interferometer_run_block.c:88-145,
+ * 240-333).  A batch holds up to max_units units, each with its own source
+ * count, station beam, fluxes, directions, station (u, v, w), frequency and
+ * output; oskar_factorised_batch_run() computes, for every unit u < num_units,
+ * exactly what oskar_cross_correlate_factorised() computes for it, in four
+ * kernel launches: stations, sources, a persistent MFMA product over all
+ * (unit, source part, tile) work items, and the FP64 reduction.  These are
+ * captured in a hipGraph on the first run and replayed by later runs with
+ * any units (sizes and pointers live in device-side descriptors).
+ *
+ * The unit inputs are referenced, not copied: they must stay valid and
+ * unchanged until the run has completed on the stream.  Units that share
+ * inputs (the directions of one time sample across channels) may pass the
+ * same oskar_Mem.  Runs on one batch are ordered by the stream; a batch
+ * belongs to the device current at creation.
+ */
+typedef struct oskar_FactorisedBatch oskar_FactorisedBatch;
+
+/* precision: OSKAR_SINGLE or OSKAR_DOUBLE (of every unit's inputs and vis). */
+OSKAR_EXPORT
+oskar_FactorisedBatch* oskar_factorised_batch_create(int precision, int mode,
+        int num_stations, int max_units, int* status);
+
+/* Arguments as oskar_cross_correlate_factorised(), for unit `unit`. */
+OSKAR_EXPORT
+void oskar_factorised_batch_set_unit(oskar_FactorisedBatch* batch, int unit,
+        int num_sources, const oskar_Mem* station_beam,
+        const oskar_Mem* const src_flux[4], const oskar_Mem* const src_dir[3],
+        const oskar_Mem* const station_uvw[3], double frequency_hz,
+        int offset_out, oskar_Mem* vis, int* status);
+
+/* Adds every unit's visibilities to its output (the uv filter and
+ * smearing checks of oskar_cross_correlate_factorised() apply, per unit). */
+OSKAR_EXPORT
+void oskar_factorised_batch_run(oskar_FactorisedBatch* batch, int num_units,
+        int use_casa_phase_convention, const oskar_Telescope* tel,
+        int* status);
+
+/* Picks the tile shape, part size and persistent grid by timing every
+ * candidate on units [0, num_units) (outputs go to scratch memory; the
+ * units' visibilities are not touched).  The result is cached per
+ * (stations, precision, mode, source-count class) for the process, so a
+ * second batch of the same kind reuses it without timing. */
+OSKAR_EXPORT
+void oskar_factorised_batch_autotune(oskar_FactorisedBatch* batch,
+        int num_units, int* status);
+
+/* Tile shapes: 0 = 128 x 64 stations, 1 = 64 x 64, 2 = 128 x 32.
+ * src_per_part: sources summed in FP32 per work item (multiple of 8).
+ * num_workgroups: persistent grid (0 = compute units x occupancy).
+ * use_graph: capture and replay a hipGraph (default 1). */
+OSKAR_EXPORT
+void oskar_factorised_batch_set_config(oskar_FactorisedBatch* batch,
+        int tile_shape, int src_per_part, int num_workgroups, int use_graph,
+        int* status);
+
+/* config[0..3] = tile shape, part size, workgroups, graph on/off. */
+OSKAR_EXPORT
+void oskar_factorised_batch_config(const oskar_FactorisedBatch* batch,
+        int config[4]);
+
+/* Milliseconds from the start of the last run's first kernel to the end of
+ * its last (waits for it); negative if there is none. */
+OSKAR_EXPORT
+double oskar_factorised_batch_last_run_ms(oskar_FactorisedBatch* batch);
+
+OSKAR_EXPORT
+void oskar_factorised_batch_free(oskar_FactorisedBatch* batch, int* status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -141,6 +141,15 @@ SIGNATURES = {
     "oskar_cross_correlate_factorised_set_part_size": (I, [I]),
     "oskar_cross_correlate_factorised_set_profiling": (None, [I]),
     "oskar_cross_correlate_factorised_last_times": (I, [DP]),
+    "oskar_factorised_batch_create": (P, [I, I, I, I, IP]),
+    "oskar_factorised_batch_set_unit": (None, [P, I, I, P, MEMA, MEM3, MEM3,
+                                               D, I, P, IP]),
+    "oskar_factorised_batch_run": (None, [P, I, I, P, IP]),
+    "oskar_factorised_batch_autotune": (None, [P, I, IP]),
+    "oskar_factorised_batch_set_config": (None, [P, I, I, I, I, IP]),
+    "oskar_factorised_batch_config": (None, [P, IP]),
+    "oskar_factorised_batch_last_run_ms": (D, [P]),
+    "oskar_factorised_batch_free": (None, [P, IP]),
     # oskar_dftw.h
     "oskar_dftw": (None, [I, I, D, P, P, P, P, I, I, P, P, P, P, P, I, I, I,
                           P, IP]),

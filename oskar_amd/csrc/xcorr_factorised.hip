@@ -21,29 +21,41 @@
  *  - P and W are split into hi + lo 16-bit parts (FP16 or BF16); each
  *    product is formed as hi*hi + hi*lo + lo*hi on the MFMA units with FP32
  *    accumulation ("x3");
- *  - K is scaled by 2^12 and C by a global power of two (from a device-side
+ *  - K is scaled by 2^12 and C by a power of two (from a device-side
  *    maximum) so the FP16 parts stay in range and the lo parts of small
  *    components stay normal (the matrix cores do not keep FP16 subnormal
  *    inputs); both scales are removed exactly at the end;
- *  - each workgroup accumulates at most SRC_PER_PART sources in FP32, and
+ *  - each workgroup accumulates at most src_per_part sources in FP32, and
  *    the per-part partial sums are added in FP64 by the reduction kernel
  *    (cross-tile sums in FP64), in a fixed order (no atomics on the data).
  *
  * Real form: with X_P = [Re K, Im K] and Y_P = [Im K, -Re K] interleaved per
  * source, X_W = [Re W, Im W],  Re V = X_P X_W^T  and  Im V = Y_P X_W^T.
  *
- * Tiling: a workgroup of 256 threads (one wave64 per SIMD) owns 128
- * P-stations x 64 Q-stations (256 W rows) = 65536 FP32 accumulators held in
- * AGPRs; each wave owns 64 P rows x 128 W rows (2 x 4 MFMA 32x32 blocks,
- * Re and Im).  A k-step is 8 sources (16 reals of K): every thread builds
- * 4 P values and 2 x 4 W values into LDS (double-buffered, 64 KB), then each
- * wave issues 48 v_mfma_f32_32x32x16 instructions.  LDS operand planes are
- * laid out [k-half][row][8] so a wave's 16-byte fragment reads are
- * contiguous; W rows are ordered (ik, q) so the builders' writes are too.
- * Source data for step k+2 is loaded into registers during step k.
+ * Tiling (template Tile<TP, TQ, OCC>): a workgroup of 256 threads (four wave64s,
+ * 2 x 2) owns TP P-stations x TQ Q-stations (TW = 4 TQ W rows) of FP32
+ * accumulators held in AGPRs; each wave owns TP/2 P rows x TW/2 W rows as
+ * RF x CF MFMA 32x32 blocks, Re and Im.  A k-step is 8 sources (16 reals of
+ * K): the threads build the P and W values of the step into LDS
+ * (double-buffered), then each wave issues 6 RF CF v_mfma_f32_32x32x16
+ * instructions.  LDS operand planes are laid out [k-half][row][8] so a wave's
+ * 16-byte fragment reads are contiguous; W rows are ordered (ik, q) so the
+ * builders' writes are too.  Source data for step k+2 is loaded into
+ * registers during step k.
+ *
+ * Launch forms sharing the tile body (tile_part):
+ *  - one call: grid (tiles, parts) over one (channel, time) unit;
+ *  - a batch: a persistent grid pulls (unit, part, tile) items from a device
+ *    counter over many units with different source counts; every size and
+ *    pointer is read from device-side unit descriptors, so the four launches
+ *    of a batch are captured once in a hipGraph and replayed.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "device_common.h"
 #include "private.h"
@@ -52,14 +64,40 @@ using namespace oskar_amd;
 
 namespace {
 
-constexpr int TP = 128;          /* P stations per tile */
-constexpr int TQ = 64;           /* Q stations per tile */
-constexpr int TW = TQ * 4;       /* W rows per tile */
 constexpr int KSRC = 8;          /* sources per k-step */
 constexpr int THREADS = 256;
-constexpr int P_ENTRIES = 4 * 2 * TP;    /* uint4 per stage, P planes */
-constexpr int W_ENTRIES = 2 * 2 * TW;    /* uint4 per stage, W planes */
-constexpr int STAGE = P_ENTRIES + W_ENTRIES;
+
+template <int TP_, int TQ_, int OCC_>
+struct Tile
+{
+    static constexpr int TP = TP_;               /* P stations per tile */
+    static constexpr int TQ = TQ_;               /* Q stations per tile */
+    static constexpr int TW = 4 * TQ_;           /* W rows per tile */
+    static constexpr int PG = THREADS / TP;      /* P builder groups */
+    static constexpr int PS = KSRC / PG;         /* sources per P builder */
+    static constexpr int QG = THREADS / TQ;      /* W builder groups */
+    static constexpr int QS = KSRC / QG;         /* sources per W builder */
+    static constexpr int P_ENTRIES = 8 * TP;     /* uint4 per stage, P */
+    static constexpr int W_ENTRIES = 4 * TW;     /* uint4 per stage, W */
+    static constexpr int STAGE = P_ENTRIES + W_ENTRIES;
+    static constexpr int RF = TP / 64;           /* MFMA rows per wave */
+    static constexpr int CF = TW / 64;           /* MFMA columns per wave */
+    static constexpr int NMFMA = 6 * RF * CF;    /* per wave per k-step */
+    static constexpr int OCC = OCC_;             /* workgroups per CU */
+    static_assert(PS >= 1 && PS <= 4 && QS >= 1 && QS <= 4, "tile shape");
+    static_assert(RF >= 1 && CF >= 1, "tile shape");
+};
+
+/* The tile shapes offered to the autotuner (index = shape code). */
+/* 256 FP32 accumulators per lane fill the AGPRs: one workgroup per CU;
+ * the 128-accumulator shapes fit two per CU when their other registers
+ * allow it without spilling. */
+using Tile0 = Tile<128, 64, 1>;
+using Tile1 = Tile<64, 64, 2>;
+using Tile2 = Tile<128, 32, 1>;
+constexpr int NUM_SHAPES = 3;
+constexpr int SHAPE_TP[NUM_SHAPES] = {128, 64, 128};
+constexpr int SHAPE_TQ[NUM_SHAPES] = {64, 64, 32};
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
@@ -67,28 +105,60 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 
 struct Src { double l, m, n1; };        /* n1 = n - 1 */
 
+/* Global-memory access through pointers the compiler cannot see are
+ * global (those read from device-side descriptors): explicit address
+ * space 1, so the loads are global_* rather than flat_* instructions (flat
+ * loads also count against the LDS wait counter). */
+#define GLOBAL_AS __attribute__((address_space(1)))
+typedef float fv4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ __forceinline__ T gld(const T* p)
+{
+    return *(const GLOBAL_AS T*) p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v)
+{
+    *(GLOBAL_AS T*) p = v;
+}
+__device__ __forceinline__ Src gld_src(const Src* p)
+{
+    const GLOBAL_AS double* q = (const GLOBAL_AS double*) p;
+    Src s;
+    s.l = q[0];
+    s.m = q[1];
+    s.n1 = q[2];
+    return s;
+}
+__device__ __forceinline__ void gst_src(Src* p, const Src& s)
+{
+    GLOBAL_AS double* q = (GLOBAL_AS double*) p;
+    q[0] = s.l;
+    q[1] = s.m;
+    q[2] = s.n1;
+}
+__device__ __forceinline__ float4 gld_f4(const float4* p)
+{
+    const fv4 v = *(const GLOBAL_AS fv4*) p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst_f4(float4* p, float x, float y, float z,
+        float w)
+{
+    fv4 v;
+    v.x = x; v.y = y; v.z = z; v.w = w;
+    *(GLOBAL_AS fv4*) p = v;
+}
+
 /* Splits x into hi + lo parts of a 16-bit type (round to nearest). */
-template <typename T16> struct Split16;
-template <> struct Split16<_Float16>
+template <typename T16>
+__device__ __forceinline__ void split16(float x, T16* hi, T16* lo)
 {
-    static __device__ __forceinline__ void split(float x, _Float16* hi,
-            _Float16* lo)
-    {
-        const _Float16 h = (_Float16) x;
-        *hi = h;
-        *lo = (_Float16) (x - (float) h);
-    }
-};
-template <> struct Split16<__bf16>
-{
-    static __device__ __forceinline__ void split(float x, __bf16* hi,
-            __bf16* lo)
-    {
-        const __bf16 h = (__bf16) x;
-        *hi = h;
-        *lo = (__bf16) (x - (float) h);
-    }
-};
+    const T16 h = (T16) x;
+    *hi = h;
+    *lo = (T16) (x - (float) h);
+}
 
 template <typename T16> __device__ __forceinline__ f16v mfma(
         const uint4& a, const uint4& b, const f16v& c);
@@ -118,22 +188,57 @@ __device__ __forceinline__ void phasor(double r, float* re, float* im)
     *im = __builtin_amdgcn_sinf(rr);
 }
 
+/* Stores CNT 16-bit values (CNT = 2, 4 or 8) at element offset off (a
+ * multiple of CNT) of the 16-byte LDS entry e. */
+template <int CNT, typename T16>
+__device__ __forceinline__ void put(uint4* e, int off, const T16* v)
+{
+    if constexpr (CNT == 8)
+    {
+        uint4 x;
+        __builtin_memcpy(&x, v, 16);
+        *e = x;
+    }
+    else if constexpr (CNT == 4)
+    {
+        uint2 x;
+        __builtin_memcpy(&x, v, 8);
+        reinterpret_cast<uint2*>(e)[off / 4] = x;
+    }
+    else
+    {
+        unsigned int x;
+        __builtin_memcpy(&x, v, 4);
+        reinterpret_cast<unsigned int*>(e)[off / 2] = x;
+    }
+}
+
 /* Number of (P-block, Q-block) tiles holding at least one p > q pair, and
  * the mapping of a linear tile index to the block pair. */
-__host__ __device__ inline int tiles_in_row(int ib, int N, int nq)
+__host__ __device__ inline int tiles_in_row(int ib, int N, int TP, int TQ)
 {
+    const int nq = (N + TQ - 1) / TQ;
     const int p_max = min(TP * ib + TP - 1, N - 1);
     if (p_max < 1) return 0;
     return min(nq, (p_max - 1) / TQ + 1);
 }
 
-__host__ __device__ inline void tile_coords(int t, int N, int* ib, int* jb)
+__host__ __device__ inline int num_tiles(int N, int TP, int TQ)
 {
-    const int np = (N + TP - 1) / TP, nq = (N + TQ - 1) / TQ;
+    const int np = (N + TP - 1) / TP;
+    int t = 0;
+    for (int i = 0; i < np; ++i) t += tiles_in_row(i, N, TP, TQ);
+    return t;
+}
+
+__host__ __device__ inline void tile_coords(int t, int N, int TP, int TQ,
+        int* ib, int* jb)
+{
+    const int np = (N + TP - 1) / TP;
     int i = 0;
     for (; i < np; ++i)
     {
-        const int c = tiles_in_row(i, N, nq);
+        const int c = tiles_in_row(i, N, TP, TQ);
         if (t < c) break;
         t -= c;
     }
@@ -141,23 +246,12 @@ __host__ __device__ inline void tile_coords(int t, int N, int* ib, int* jb)
     *jb = t;
 }
 
-__host__ __device__ inline int tile_index(int ib, int jb, int N)
+__host__ __device__ inline int row_first_tile(int ib, int N, int TP, int TQ)
 {
-    const int nq = (N + TQ - 1) / TQ;
     int t = 0;
-    for (int i = 0; i < ib; ++i) t += tiles_in_row(i, N, nq);
-    return t + jb;
+    for (int i = 0; i < ib; ++i) t += tiles_in_row(i, N, TP, TQ);
+    return t;
 }
-
-struct FactArgs
-{
-    int num_sources, num_stations, num_tiles, src_per_part;
-    const Src* src;              /* per source: l, m, n - 1 (FP64) */
-    const float4* cmat;          /* per source: C00, Re C01, Im C01, C11 */
-    const double* st_uvw;        /* per station: U, V, W (revolutions) */
-    const unsigned int* cmax;    /* bit pattern of max |C| (>= 0 float) */
-    float* partial;              /* [part][tile][re/im][TP][TW] */
-};
 
 /* Scale of the K operand: |K| <= 1 -> <= 2^12, so lo parts of components
  * down to 2^-15 stay normal FP16 numbers. */
@@ -166,107 +260,119 @@ constexpr float K_SCALE = 4096.0f;
 /* Power-of-two scale that puts max |C| just below 2^14. */
 __device__ __forceinline__ float c_scale(const unsigned int* cmax)
 {
-    const float m = __uint_as_float(*cmax);
+    const float m = __uint_as_float(gld(cmax));
     if (!(m > 0.0f) || !isfinite(m)) return 1.0f;
     int e = 0;
     (void) frexpf(m, &e);          /* m = f * 2^e, 0.5 <= f < 1 */
     return ldexpf(1.0f, 14 - e);
 }
 
-/* Source data one thread needs for one k-step: 4 sources for its P row,
- * 2 sources (with C) for its W rows.  Loaded two k-steps ahead so the
+/* Source data one thread needs for one k-step: PS sources for its P row,
+ * QS sources (with C) for its W rows.  Loaded two k-steps ahead so the
  * global-memory latency hides behind the MFMAs of the current step. */
+template <int PS, int QS>
 struct StepSrc
 {
-    Src p[4];
-    Src w[2];
-    float4 c[2];
+    Src p[PS];
+    Src w[QS];
+    float4 c[QS];
 };
 
-__device__ __forceinline__ void load_step(const FactArgs& A, int sb,
-        int s_last, int pg, int qg, StepSrc* o)
+/* One (tile, source part) of the product: the FP32 sums over sources
+ * [s_begin, s_end) for P stations p0.. and Q stations q0.., stored to out as
+ * [re/im][TP][TW] (W rows ordered ik * TQ + q). */
+struct TileJob
+{
+    int N, p0, q0, s_begin, s_end;
+    const Src* src;              /* per source: l, m, n - 1 (FP64) */
+    const float4* cmat;          /* per source: C00, Re C01, Im C01, C11 */
+    const double* st_uvw;        /* per station: U, V, W (revolutions) */
+    float scale;                 /* power-of-two scale of C */
+    float* out;
+};
+
+template <int PS, int QS>
+__device__ __forceinline__ void load_step(const TileJob& J, int sb,
+        int s_last, int pg, int qg, StepSrc<PS, QS>* o)
 {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-        o->p[j] = A.src[min(sb + pg * 4 + j, s_last)];
+    for (int j = 0; j < PS; ++j)
+        o->p[j] = gld_src(J.src + min(sb + pg * PS + j, s_last));
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < QS; ++j)
     {
-        const int s = min(sb + qg * 2 + j, s_last);
-        o->w[j] = A.src[s];
-        o->c[j] = A.cmat[s];
+        const int s = min(sb + qg * QS + j, s_last);
+        o->w[j] = gld_src(J.src + s);
+        o->c[j] = gld_f4(J.cmat + s);
     }
 }
 
-template <typename T16>
-__global__ __launch_bounds__(THREADS, 1)
-void fact_xcorr_kernel(FactArgs A)
+template <typename T16, class TL>
+__device__ __forceinline__ void tile_part(const TileJob& J,
+        uint4 (*lds)[TL::STAGE])
 {
-    __shared__ uint4 lds[2][STAGE];
-    const int N = A.num_stations;
-    const int tile = blockIdx.x;
-    const int part = blockIdx.y;
-    int ib, jb;
-    tile_coords(tile, N, &ib, &jb);
-    const int p0 = ib * TP, q0 = jb * TQ;
+    constexpr int TP = TL::TP, TQ = TL::TQ, TW = TL::TW;
+    constexpr int PS = TL::PS, QS = TL::QS, RF = TL::RF, CF = TL::CF;
+    constexpr int P_ENTRIES = TL::P_ENTRIES;
+    const int N = J.N;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const float scale = c_scale(A.cmax);
 
     /* Station constants for this thread's P and W builds; stations past N
      * get a zero mask (their rows build as zeros). */
-    const int pb = tid & (TP - 1), pg = tid >> 7;     /* P: station, half */
-    const int qb = tid & (TQ - 1), qg = tid >> 6;     /* W: station, group */
-    const int ps = min(p0 + pb, N - 1), qs = min(q0 + qb, N - 1);
-    const double pu = A.st_uvw[3 * ps], pv = A.st_uvw[3 * ps + 1],
-            pw = A.st_uvw[3 * ps + 2];
-    const double qu = A.st_uvw[3 * qs], qv = A.st_uvw[3 * qs + 1],
-            qw = A.st_uvw[3 * qs + 2];
-    const float p_mask = (p0 + pb < N) ? K_SCALE : 0.0f;
-    const float q_mask = (q0 + qb < N) ? scale : 0.0f;
+    const int pb = tid % TP, pg = tid / TP;       /* P: station, group */
+    const int qb = tid % TQ, qg = tid / TQ;       /* W: station, group */
+    const int ps = min(J.p0 + pb, N - 1), qs = min(J.q0 + qb, N - 1);
+    const double pu = gld(J.st_uvw + 3 * ps), pv = gld(J.st_uvw + 3 * ps + 1),
+            pw = gld(J.st_uvw + 3 * ps + 2);
+    const double qu = gld(J.st_uvw + 3 * qs), qv = gld(J.st_uvw + 3 * qs + 1),
+            qw = gld(J.st_uvw + 3 * qs + 2);
+    const float p_mask = (J.p0 + pb < N) ? K_SCALE : 0.0f;
+    const float q_mask = (J.q0 + qb < N) ? J.scale : 0.0f;
+    /* Position of this thread's sources in the k-step: plane half and
+     * 16-bit element offset within the 16-byte entry. */
+    const int p_half = (pg * PS) / 4, p_off = ((pg * PS) % 4) * 2;
+    const int q_half = (qg * QS) / 4, q_off = ((qg * QS) % 4) * 2;
 
-    const int s_begin = part * A.src_per_part;
-    const int s_end = min(A.num_sources, s_begin + A.src_per_part);
-    const int s_last = s_end - 1;
-    const int nsteps = (s_end - s_begin + KSRC - 1) / KSRC;
+    const int s_begin = J.s_begin, s_last = J.s_end - 1;
+    const int nsteps = (J.s_end - s_begin + KSRC - 1) / KSRC;
 
     /* Builds the P and W operand planes of the k-step starting at source sb
      * from the prefetched source data, into one stage buffer.  Branch-free:
      * sources past the end of this part are masked to zero. */
-    auto build = [&](int sb, const StepSrc& d, uint4* st) {
+    auto build = [&](int sb, const StepSrc<PS, QS>& d, uint4* st) {
         {
-            T16 xh[8], xl[8], yh[8], yl[8];
+            T16 xh[2 * PS], xl[2 * PS], yh[2 * PS], yl[2 * PS];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < PS; ++j)
             {
-                const float mk = (sb + pg * 4 + j <= s_last) ? p_mask : 0.0f;
+                const float mk = (sb + pg * PS + j <= s_last) ? p_mask : 0.0f;
                 float re, im;
                 phasor(pu * d.p[j].l + pv * d.p[j].m + pw * d.p[j].n1, &re,
                         &im);
                 re *= mk;
                 im *= mk;
-                Split16<T16>::split(re, &xh[2 * j], &xl[2 * j]);
-                Split16<T16>::split(im, &xh[2 * j + 1], &xl[2 * j + 1]);
+                split16<T16>(re, &xh[2 * j], &xl[2 * j]);
+                split16<T16>(im, &xh[2 * j + 1], &xl[2 * j + 1]);
                 /* Y = (Im, -Re): negation is exact in the split parts. */
                 yh[2 * j] = xh[2 * j + 1];
                 yl[2 * j] = xl[2 * j + 1];
                 yh[2 * j + 1] = -xh[2 * j];
                 yl[2 * j + 1] = -xl[2 * j];
             }
-            uint4 v;
-            __builtin_memcpy(&v, xh, 16); st[(0 * 2 + pg) * TP + pb] = v;
-            __builtin_memcpy(&v, xl, 16); st[(1 * 2 + pg) * TP + pb] = v;
-            __builtin_memcpy(&v, yh, 16); st[(2 * 2 + pg) * TP + pb] = v;
-            __builtin_memcpy(&v, yl, 16); st[(3 * 2 + pg) * TP + pb] = v;
+            put<2 * PS>(&st[(0 * 2 + p_half) * TP + pb], p_off, xh);
+            put<2 * PS>(&st[(1 * 2 + p_half) * TP + pb], p_off, xl);
+            put<2 * PS>(&st[(2 * 2 + p_half) * TP + pb], p_off, yh);
+            put<2 * PS>(&st[(3 * 2 + p_half) * TP + pb], p_off, yl);
         }
         {
-            /* W rows are ordered ik-major (row = ik * TQ + q), so the 64
-             * threads of a wave write 64 consecutive rows. */
-            T16 wh[4][4], wl[4][4];
+            /* W rows are ordered ik-major (row = ik * TQ + q), so the
+             * threads of a wave write consecutive rows. */
+            T16 wh[4][2 * QS], wl[4][2 * QS];
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < QS; ++j)
             {
-                const float mk = (sb + qg * 2 + j <= s_last) ? q_mask : 0.0f;
+                const float mk = (sb + qg * QS + j <= s_last) ? q_mask : 0.0f;
                 float kr, ki;
                 phasor(qu * d.w[j].l + qv * d.w[j].m + qw * d.w[j].n1, &kr,
                         &ki);
@@ -281,44 +387,39 @@ void fact_xcorr_kernel(FactArgs A)
 #pragma unroll
                 for (int ik = 0; ik < 4; ++ik)
                 {
-                    Split16<T16>::split(w[ik][0], &wh[ik][2 * j],
-                            &wl[ik][2 * j]);
-                    Split16<T16>::split(w[ik][1], &wh[ik][2 * j + 1],
+                    split16<T16>(w[ik][0], &wh[ik][2 * j], &wl[ik][2 * j]);
+                    split16<T16>(w[ik][1], &wh[ik][2 * j + 1],
                             &wl[ik][2 * j + 1]);
                 }
             }
-            const int half = qg >> 1, off = (qg & 1);
 #pragma unroll
             for (int ik = 0; ik < 4; ++ik)
             {
                 const int row = ik * TQ + qb;
-                uint2 v;
-                __builtin_memcpy(&v, wh[ik], 8);
-                reinterpret_cast<uint2*>(
-                        &st[P_ENTRIES + (0 * 2 + half) * TW + row])[off] = v;
-                __builtin_memcpy(&v, wl[ik], 8);
-                reinterpret_cast<uint2*>(
-                        &st[P_ENTRIES + (1 * 2 + half) * TW + row])[off] = v;
+                put<2 * QS>(&st[P_ENTRIES + (0 * 2 + q_half) * TW + row],
+                        q_off, wh[ik]);
+                put<2 * QS>(&st[P_ENTRIES + (1 * 2 + q_half) * TW + row],
+                        q_off, wl[ik]);
             }
         }
     };
 
-    f16v re[2][4], im[2][4];
+    f16v re[RF][CF], im[RF][CF];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < RF; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int b = 0; b < CF; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) { re[a][b][v] = 0.0f; im[a][b][v] = 0.0f; }
 
     const int wp = wave & 1, ww = wave >> 1;
     const int lr = lane & 31, lh = lane >> 5;
-    StepSrc cur;
+    StepSrc<PS, QS> cur;
     if (nsteps > 0)
     {
-        load_step(A, s_begin, s_last, pg, qg, &cur);
+        load_step(J, s_begin, s_last, pg, qg, &cur);
         build(s_begin, cur, lds[0]);
-        load_step(A, s_begin + KSRC, s_last, pg, qg, &cur);
+        load_step(J, s_begin + KSRC, s_last, pg, qg, &cur);
     }
     __syncthreads();
     for (int ks = 0; ks < nsteps; ++ks)
@@ -327,45 +428,45 @@ void fact_xcorr_kernel(FactArgs A)
         /* Operand fragments of step ks first: the builder's LDS writes to
          * the other stage come after them in program order, so the MFMAs
          * need not wait for the build. */
-        uint4 xh[2], xl[2], yh[2], yl[2], wh[4], wl[4];
+        uint4 xh[RF], xl[RF], yh[RF], yl[RF], wh[CF], wl[CF];
 #pragma unroll
-        for (int rf = 0; rf < 2; ++rf)
+        for (int rf = 0; rf < RF; ++rf)
         {
-            const int row = wp * 64 + rf * 32 + lr;
+            const int row = wp * (TP / 2) + rf * 32 + lr;
             xh[rf] = st[(0 * 2 + lh) * TP + row];
             xl[rf] = st[(1 * 2 + lh) * TP + row];
             yh[rf] = st[(2 * 2 + lh) * TP + row];
             yl[rf] = st[(3 * 2 + lh) * TP + row];
         }
 #pragma unroll
-        for (int cf = 0; cf < 4; ++cf)
+        for (int cf = 0; cf < CF; ++cf)
         {
-            const int row = ww * 128 + cf * 32 + lr;
+            const int row = ww * (TW / 2) + cf * 32 + lr;
             wh[cf] = st[P_ENTRIES + (0 * 2 + lh) * TW + row];
             wl[cf] = st[P_ENTRIES + (1 * 2 + lh) * TW + row];
         }
-        /* hi*hi, then hi*lo, then lo*hi: 16 independent MFMAs between
+        /* hi*hi, then hi*lo, then lo*hi: 2 RF CF independent MFMAs between
          * successive updates of one accumulator. */
 #pragma unroll
-        for (int rf = 0; rf < 2; ++rf)
+        for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
-            for (int cf = 0; cf < 4; ++cf)
+            for (int cf = 0; cf < CF; ++cf)
             {
                 re[rf][cf] = mfma<T16>(xh[rf], wh[cf], re[rf][cf]);
                 im[rf][cf] = mfma<T16>(yh[rf], wh[cf], im[rf][cf]);
             }
 #pragma unroll
-        for (int rf = 0; rf < 2; ++rf)
+        for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
-            for (int cf = 0; cf < 4; ++cf)
+            for (int cf = 0; cf < CF; ++cf)
             {
                 re[rf][cf] = mfma<T16>(xh[rf], wl[cf], re[rf][cf]);
                 im[rf][cf] = mfma<T16>(yh[rf], wl[cf], im[rf][cf]);
             }
 #pragma unroll
-        for (int rf = 0; rf < 2; ++rf)
+        for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
-            for (int cf = 0; cf < 4; ++cf)
+            for (int cf = 0; cf < CF; ++cf)
             {
                 re[rf][cf] = mfma<T16>(xl[rf], wh[cf], re[rf][cf]);
                 im[rf][cf] = mfma<T16>(yl[rf], wh[cf], im[rf][cf]);
@@ -373,98 +474,99 @@ void fact_xcorr_kernel(FactArgs A)
         /* Build step ks+1 (a masked dummy after the last step) into the
          * other stage, then start the loads for step ks+2. */
         build(s_begin + (ks + 1) * KSRC, cur, lds[(ks + 1) & 1]);
-        load_step(A, s_begin + (ks + 2) * KSRC, s_last, pg, qg, &cur);
+        load_step(J, s_begin + (ks + 2) * KSRC, s_last, pg, qg, &cur);
         /* Schedule: operand reads, then each MFMA followed by a few VALU
          * instructions of the build, then the builder's LDS writes and the
          * next loads.  The matrix pipe runs while the VALU builds. */
-        __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 4 * RF + 2 * CF, 0);
 #pragma unroll
-        for (int i = 0; i < 48; ++i)
+        for (int i = 0; i < TL::NMFMA; ++i)
         {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 192 / TL::NMFMA, 0);
         }
-        __builtin_amdgcn_sched_group_barrier(0x200, 8, 0);
-        __builtin_amdgcn_sched_group_barrier(0x020, 14, 0);
+        __builtin_amdgcn_sched_group_barrier(0x200, 12, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 2 * PS + 3 * QS, 0);
         __syncthreads();
     }
 
     /* Store this part's FP32 tile: element (row i, col j) of a 32x32 block
      * sits in lane (j + 32 * ((i / 4) % 2)), register 4 * (i / 8) + i % 4.
      * Columns are W rows, ordered ik * TQ + q. */
-    float* base = A.partial + ((size_t) part * A.num_tiles + tile) *
-            (size_t) (2 * TP * TW);
+    float* base = J.out;
 #pragma unroll
-    for (int rf = 0; rf < 2; ++rf)
+    for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
-        for (int cf = 0; cf < 4; ++cf)
+        for (int cf = 0; cf < CF; ++cf)
 #pragma unroll
             for (int v = 0; v < 16; ++v)
             {
                 const int i = 8 * (v / 4) + 4 * lh + (v % 4);
-                const int row = wp * 64 + rf * 32 + i;
-                const int col = ww * 128 + cf * 32 + lr;
-                base[(size_t) row * TW + col] = re[rf][cf][v];
-                base[(size_t) (TP * TW) + (size_t) row * TW + col] =
-                        im[rf][cf][v];
+                const int row = wp * (TP / 2) + rf * 32 + i;
+                const int col = ww * (TW / 2) + cf * 32 + lr;
+                gst(base + (size_t) row * TW + col, (float) re[rf][cf][v]);
+                gst(base + (size_t) (TP * TW) + (size_t) row * TW + col,
+                        (float) im[rf][cf][v]);
             }
 }
 
-/* Per-source preparation: direction cosines in FP64 and C = E B E^H, with
- * the running maximum of |C| entries (order-independent atomic max on the
- * non-negative float bit pattern). */
+/* ---------------------------------------------------------------------- */
+/* Per-source and per-station preparation (shared by both launch forms).  */
+
+/* Writes src[s] and cmat[s] = C = E B E^H (packed as C00, Re C01, Im C01,
+ * C11) for one source; returns max |C entry|. */
 template <typename FP>
-__global__ __launch_bounds__(256)
-void fact_prepare_kernel(int num_sources, const FP* I, const FP* Q,
-        const FP* U, const FP* V, const FP* l, const FP* m, const FP* n,
-        const Cplx<FP>* beam, Src* src, float4* cmat, unsigned int* cmax)
+__device__ __forceinline__ float prepare_source(int s, const FP* I,
+        const FP* Q, const FP* U, const FP* V, const FP* l, const FP* m,
+        const FP* n, const Cplx<FP>* beam, Src* src, float4* cmat)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    float mx = 0.0f;
-    if (s < num_sources)
+    Src o;
+    o.l = (double) gld(l + s);
+    o.m = (double) gld(m + s);
+    o.n1 = (double) gld(n + s) - 1.0;
+    gst_src(src + s, o);
+    /* B = [[I+Q, U+iV], [U-iV, I-Q]] */
+    const double Ii = gld(I + s), Qi = Q ? (double) gld(Q + s) : 0.0;
+    const double Ui = U ? (double) gld(U + s) : 0.0;
+    const double Vi = V ? (double) gld(V + s) : 0.0;
+    double c00, c01r, c01i, c11;
+    if (beam)
     {
-        Src o;
-        o.l = (double) l[s];
-        o.m = (double) m[s];
-        o.n1 = (double) n[s] - 1.0;
-        src[s] = o;
-        /* B = [[I+Q, U+iV], [U-iV, I-Q]] */
-        const double Ii = I[s], Qi = Q ? (double) Q[s] : 0.0;
-        const double Ui = U ? (double) U[s] : 0.0;
-        const double Vi = V ? (double) V[s] : 0.0;
-        double c00, c01r, c01i, c11;
-        if (beam)
-        {
-            const Cplx<FP>* e = beam + 4 * (size_t) s;
-            const double ar = e[0].x, ai = e[0].y, br = e[1].x, bi = e[1].y;
-            const double cr = e[2].x, ci = e[2].y, dr = e[3].x, di = e[3].y;
-            const double ba = Ii + Qi, bd = Ii - Qi;
-            /* M = E B: row 0 = (a, b), row 1 = (c, d). */
-            const double m00r = ar * ba + (br * Ui + bi * Vi);
-            const double m00i = ai * ba + (bi * Ui - br * Vi);
-            const double m01r = br * bd + (ar * Ui - ai * Vi);
-            const double m01i = bi * bd + (ar * Vi + ai * Ui);
-            const double m10r = cr * ba + (dr * Ui + di * Vi);
-            const double m10i = ci * ba + (di * Ui - dr * Vi);
-            const double m11r = dr * bd + (cr * Ui - ci * Vi);
-            const double m11i = di * bd + (cr * Vi + ci * Ui);
-            /* C = M E^H: C_ik = M_i0 conj(E_k0) + M_i1 conj(E_k1). */
-            c00 = m00r * ar + m00i * ai + m01r * br + m01i * bi;
-            c01r = m00r * cr + m00i * ci + m01r * dr + m01i * di;
-            c01i = m00i * cr - m00r * ci + m01i * dr - m01r * di;
-            c11 = m10r * cr + m10i * ci + m11r * dr + m11i * di;
-        }
-        else
-        {
-            c00 = Ii + Qi; c01r = Ui; c01i = Vi; c11 = Ii - Qi;
-        }
-        cmat[s] = make_float4((float) c00, (float) c01r, (float) c01i,
-                (float) c11);
-        mx = fmaxf(fmaxf(fabsf((float) c00), fabsf((float) c11)),
-                fmaxf(fabsf((float) c01r), fabsf((float) c01i)));
+        const FP* e = (const FP*) (beam + 4 * (size_t) s);
+        const double ar = gld(e), ai = gld(e + 1), br = gld(e + 2),
+                bi = gld(e + 3);
+        const double cr = gld(e + 4), ci = gld(e + 5), dr = gld(e + 6),
+                di = gld(e + 7);
+        const double ba = Ii + Qi, bd = Ii - Qi;
+        /* M = E B: row 0 = (a, b), row 1 = (c, d). */
+        const double m00r = ar * ba + (br * Ui + bi * Vi);
+        const double m00i = ai * ba + (bi * Ui - br * Vi);
+        const double m01r = br * bd + (ar * Ui - ai * Vi);
+        const double m01i = bi * bd + (ar * Vi + ai * Ui);
+        const double m10r = cr * ba + (dr * Ui + di * Vi);
+        const double m10i = ci * ba + (di * Ui - dr * Vi);
+        const double m11r = dr * bd + (cr * Ui - ci * Vi);
+        const double m11i = di * bd + (cr * Vi + ci * Ui);
+        /* C = M E^H: C_ik = M_i0 conj(E_k0) + M_i1 conj(E_k1). */
+        c00 = m00r * ar + m00i * ai + m01r * br + m01i * bi;
+        c01r = m00r * cr + m00i * ci + m01r * dr + m01i * di;
+        c01i = m00i * cr - m00r * ci + m01i * dr - m01r * di;
+        c11 = m10r * cr + m10i * ci + m11r * dr + m11i * di;
     }
-    /* Workgroup max, then one atomic per workgroup. */
-    __shared__ float red[256];
+    else
+    {
+        c00 = Ii + Qi; c01r = Ui; c01i = Vi; c11 = Ii - Qi;
+    }
+    gst_f4(cmat + s, (float) c00, (float) c01r, (float) c01i, (float) c11);
+    return fmaxf(fmaxf(fabsf((float) c00), fabsf((float) c11)),
+            fmaxf(fabsf((float) c01r), fabsf((float) c01i)));
+}
+
+/* Workgroup maximum of non-negative values, then one atomic per workgroup
+ * (order-independent: atomic max on the float bit pattern). */
+__device__ __forceinline__ void block_max_to(float mx, unsigned int* cmax,
+        float* red)
+{
     red[threadIdx.x] = mx;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1)
@@ -474,7 +576,109 @@ void fact_prepare_kernel(int num_sources, const FP* I, const FP* Q,
         __syncthreads();
     }
     if (threadIdx.x == 0 && red[0] > 0.0f)
-        atomicMax(cmax, __float_as_uint(red[0]));
+        (void) __hip_atomic_fetch_max((GLOBAL_AS unsigned int*) cmax,
+                __float_as_uint(red[0]), __ATOMIC_RELAXED,
+                __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* Sum over parts in FP64 of one station row p (q from thread, stepping by
+ * the workgroup size), scale removal, and accumulation into
+ * vis[offset_out + b] for baselines inside the uv filter.  Consecutive
+ * threads read consecutive partials. */
+template <typename FP, class TL>
+__device__ __forceinline__ void reduce_row(int N, int p, int tiles,
+        int parts, int use_casa, const float* partial,
+        const unsigned int* cmax, const double* st_uvw, double uv_min,
+        double uv_max, int offset_out, Cplx<FP>* vis)
+{
+    constexpr int TP = TL::TP, TQ = TL::TQ, TW = TL::TW;
+    const int ib = p / TP, pl = p - ib * TP;
+    const int t0 = row_first_tile(ib, N, TP, TQ);
+    const double inv = 1.0 / ((double) c_scale(cmax) * (double) K_SCALE);
+    for (int q = threadIdx.x; q < p; q += blockDim.x)
+    {
+        /* uv filter on the baseline length in wavelengths. */
+        const double du = gld(st_uvw + 3 * p) - gld(st_uvw + 3 * q);
+        const double dv = gld(st_uvw + 3 * p + 1) - gld(st_uvw + 3 * q + 1);
+        const double len = sqrt(du * du + dv * dv);
+        if (len < uv_min || len > uv_max) continue;
+        const int jb = q / TQ, ql = q - jb * TQ;
+        const int t = t0 + jb;
+        double vr[4] = {0, 0, 0, 0}, vi[4] = {0, 0, 0, 0};
+        for (int k = 0; k < parts; ++k)
+        {
+            const float* b = partial + ((size_t) k * tiles + t) *
+                    (size_t) (2 * TP * TW) + (size_t) pl * TW + ql;
+#pragma unroll
+            for (int ik = 0; ik < 4; ++ik)
+            {
+                vr[ik] += (double) gld(b + ik * TQ);
+                vi[ik] += (double) gld(b + TP * TW + ik * TQ);
+            }
+        }
+        const int bl = q * (N - 1) - (q - 1) * q / 2 + p - q - 1;
+        FP* o = (FP*) (vis + 4 * (size_t) (offset_out + bl));
+#pragma unroll
+        for (int ik = 0; ik < 4; ++ik)
+        {
+            /* CASA: V = (J_p B J_q^H)^H -> element [i][k] = conj(V[k][i]). */
+            const int sidx = use_casa ? ((ik & 1) * 2 + (ik >> 1)) : ik;
+            const double r = vr[sidx] * inv;
+            const double i = (use_casa ? -vi[sidx] : vi[sidx]) * inv;
+            gst(o + 2 * ik, (FP) (gld(o + 2 * ik) + (FP) r));
+            gst(o + 2 * ik + 1, (FP) (gld(o + 2 * ik + 1) + (FP) i));
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* One call: one (channel, time) unit.                                     */
+
+struct FactArgs
+{
+    int num_sources, num_stations, num_tiles, src_per_part;
+    const Src* src;
+    const float4* cmat;
+    const double* st_uvw;
+    const unsigned int* cmax;    /* bit pattern of max |C| (>= 0 float) */
+    float* partial;              /* [part][tile][re/im][TP][TW] */
+};
+
+template <typename T16, class TL>
+__global__ __launch_bounds__(THREADS, TL::OCC)
+void fact_xcorr_kernel(FactArgs A)
+{
+    __shared__ uint4 lds[2][TL::STAGE];
+    const int tile = blockIdx.x, part = blockIdx.y;
+    int ib, jb;
+    tile_coords(tile, A.num_stations, TL::TP, TL::TQ, &ib, &jb);
+    TileJob J;
+    J.N = A.num_stations;
+    J.p0 = ib * TL::TP;
+    J.q0 = jb * TL::TQ;
+    J.s_begin = part * A.src_per_part;
+    J.s_end = min(A.num_sources, J.s_begin + A.src_per_part);
+    J.src = A.src;
+    J.cmat = A.cmat;
+    J.st_uvw = A.st_uvw;
+    J.scale = c_scale(A.cmax);
+    J.out = A.partial + ((size_t) part * A.num_tiles + tile) *
+            (size_t) (2 * TL::TP * TL::TW);
+    tile_part<T16, TL>(J, lds);
+}
+
+template <typename FP>
+__global__ __launch_bounds__(256)
+void fact_prepare_kernel(int num_sources, const FP* I, const FP* Q,
+        const FP* U, const FP* V, const FP* l, const FP* m, const FP* n,
+        const Cplx<FP>* beam, Src* src, float4* cmat, unsigned int* cmax)
+{
+    __shared__ float red[256];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    float mx = 0.0f;
+    if (s < num_sources)
+        mx = prepare_source<FP>(s, I, Q, U, V, l, m, n, beam, src, cmat);
+    block_max_to(mx, cmax, red);
 }
 
 /* Station phase constants U, V, W = (u, v, w) f / c (FP64). */
@@ -492,54 +696,166 @@ __global__ void fact_stations_kernel(int n, const FP* u, const FP* v,
     }
 }
 
-/* Sum of the per-part tiles in FP64, scale removal, and accumulation into
- * vis[offset_out + b] for p > q outside the uv filter.  Thread per (p, q):
- * grid y = p, x over q, so consecutive threads read consecutive partials. */
-template <typename FP>
+template <typename FP, class TL>
 __global__ __launch_bounds__(256)
 void fact_reduce_kernel(int N, int num_tiles, int num_parts, int use_casa,
         const float* partial, const unsigned int* cmax, const double* st_uvw,
         double uv_min, double uv_max, int offset_out, Cplx<FP>* vis)
 {
-    const int p = blockIdx.y;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= p) return;
-    /* uv filter on the baseline length in wavelengths. */
+    reduce_row<FP, TL>(N, blockIdx.x, num_tiles, num_parts, use_casa,
+            partial, cmax, st_uvw, uv_min, uv_max, offset_out, vis);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Batches: many units, one persistent launch, descriptors in HBM.        */
+
+struct BatchHeader
+{
+    int num_units, num_stations, num_tiles, src_per_part;
+    int total_items, prep_total, use_casa, pad0;
+};
+
+struct UnitDesc
+{
+    const void *I, *Q, *U, *V, *l, *m, *n, *beam, *st_u, *st_v, *st_w;
+    void* vis;
+    Src* src;
+    float4* cmat;
+    double* st;
+    unsigned int* cmax;
+    float* partial;
+    double f_over_c;
+    double uv_min, uv_max;       /* uv filter (wavelengths) */
+    int num_sources, offset_out;
+    int item_start;              /* first work item (prefix over units) */
+    int num_parts;
+    int prep_start;              /* first prepare slot (256-aligned) */
+    int pad0;
+};
+
+/* Last unit u with key(u) <= x (keys non-decreasing; key(0) = 0 <= x).
+ * Uniform across the workgroup: the loads are scalar. */
+template <class F>
+__device__ __forceinline__ int find_unit(int nu, int x, F key)
+{
+    int lo = 0, hi = nu - 1;
+    while (lo < hi)
     {
-        const double du = st_uvw[3 * p] - st_uvw[3 * q];
-        const double dv = st_uvw[3 * p + 1] - st_uvw[3 * q + 1];
-        const double len = sqrt(du * du + dv * dv);
-        if (len < uv_min || len > uv_max) return;
+        const int mid = (lo + hi + 1) >> 1;
+        if (key(mid) <= x) lo = mid;
+        else hi = mid - 1;
     }
-    const int ib = p / TP, jb = q / TQ;
-    const int t = tile_index(ib, jb, N);
-    const int pl = p - ib * TP, ql = q - jb * TQ;
-    double vr[4] = {0, 0, 0, 0}, vi[4] = {0, 0, 0, 0};
-    for (int k = 0; k < num_parts; ++k)
+    return lo;
+}
+
+/* Station constants of every unit; resets each unit's |C| maximum and the
+ * work-item counter (before the product kernel, in stream order). */
+template <typename FP>
+__global__ __launch_bounds__(256)
+void fact_batch_stations_kernel(const BatchHeader* H, const UnitDesc* D,
+        int* counter)
+{
+    const int N = H->num_stations, total = H->num_units * N;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total;
+            i += gridDim.x * blockDim.x)
     {
-        const float* b = partial + ((size_t) k * num_tiles + t) *
-                (size_t) (2 * TP * TW) + (size_t) pl * TW + ql;
-#pragma unroll
-        for (int ik = 0; ik < 4; ++ik)
-        {
-            vr[ik] += (double) b[ik * TQ];
-            vi[ik] += (double) b[TP * TW + ik * TQ];
-        }
+        const int u = i / N, k = i - u * N;
+        const UnitDesc& d = D[u];
+        const double f = d.f_over_c;
+        gst(d.st + 3 * k, (double) gld((const FP*) d.st_u + k) * f);
+        gst(d.st + 3 * k + 1, (double) gld((const FP*) d.st_v + k) * f);
+        gst(d.st + 3 * k + 2, (double) gld((const FP*) d.st_w + k) * f);
+        if (k == 0) gst(d.cmax, 0u);
     }
-    const double inv = 1.0 / ((double) c_scale(cmax) * (double) K_SCALE);
-    const int bl = q * (N - 1) - (q - 1) * q / 2 + p - q - 1;
-    Cplx<FP>* o = vis + 4 * (size_t) (offset_out + bl);
-#pragma unroll
-    for (int ik = 0; ik < 4; ++ik)
+    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
+}
+
+/* Sources of every unit: slots of 256 per workgroup; a unit's slots start
+ * 256-aligned, so one workgroup never straddles two units. */
+template <typename FP>
+__global__ __launch_bounds__(256)
+void fact_batch_prepare_kernel(const BatchHeader* H, const UnitDesc* D)
+{
+    __shared__ float red[256];
+    const int nu = H->num_units, total = H->prep_total;
+    for (int c = blockIdx.x * 256; c < total; c += gridDim.x * 256)
     {
-        /* CASA: V = (J_p B J_q^H)^H -> element [i][k] = conj(V[k][i]). */
-        const int src = use_casa ? ((ik & 1) * 2 + (ik >> 1)) : ik;
-        const double r = vr[src] * inv;
-        const double i = (use_casa ? -vi[src] : vi[src]) * inv;
-        o[ik].x += (FP) r;
-        o[ik].y += (FP) i;
+        const int u = find_unit(nu, c,
+                [&](int k) { return D[k].prep_start; });
+        const UnitDesc& d = D[u];
+        const int s = c - d.prep_start + (int) threadIdx.x;
+        float mx = 0.0f;
+        if (s < d.num_sources)
+            mx = prepare_source<FP>(s, (const FP*) d.I, (const FP*) d.Q,
+                    (const FP*) d.U, (const FP*) d.V, (const FP*) d.l,
+                    (const FP*) d.m, (const FP*) d.n,
+                    (const Cplx<FP>*) d.beam, d.src, d.cmat);
+        block_max_to(mx, d.cmax, red);
+        __syncthreads();
     }
 }
+
+/* Persistent product: each workgroup takes (unit, part, tile) items from
+ * the device counter until all are done (every workgroup exits when the
+ * counter passes total_items).  Items of one part are consecutive, so
+ * concurrently running workgroups share source data in L2. */
+template <typename T16, class TL>
+__global__ __launch_bounds__(THREADS, TL::OCC)
+void fact_batch_kernel(const BatchHeader* H, const UnitDesc* D, int* counter)
+{
+    __shared__ uint4 lds[2][TL::STAGE];
+    __shared__ int s_item;
+    const int total = H->total_items, nu = H->num_units;
+    const int N = H->num_stations, tiles = H->num_tiles;
+    const int spp = H->src_per_part;
+    for (;;)
+    {
+        __syncthreads();          /* every thread has read s_item */
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int item = s_item;
+        if (item >= total) break;
+        const int u = find_unit(nu, item,
+                [&](int k) { return D[k].item_start; });
+        const UnitDesc& d = D[u];
+        const int local = item - d.item_start;
+        const int part = local / tiles, tile = local - part * tiles;
+        int ib, jb;
+        tile_coords(tile, N, TL::TP, TL::TQ, &ib, &jb);
+        TileJob J;
+        J.N = N;
+        J.p0 = ib * TL::TP;
+        J.q0 = jb * TL::TQ;
+        J.s_begin = part * spp;
+        J.s_end = min(d.num_sources, J.s_begin + spp);
+        J.src = d.src;
+        J.cmat = d.cmat;
+        J.st_uvw = d.st;
+        J.scale = c_scale(d.cmax);
+        J.out = d.partial + ((size_t) part * tiles + tile) *
+                (size_t) (2 * TL::TP * TL::TW);
+        tile_part<T16, TL>(J, lds);
+    }
+}
+
+template <typename FP, class TL>
+__global__ __launch_bounds__(256)
+void fact_batch_reduce_kernel(const BatchHeader* H, const UnitDesc* D)
+{
+    const int N = H->num_stations, rows = H->num_units * N;
+    for (int r = blockIdx.x; r < rows; r += gridDim.x)
+    {
+        const int u = r / N, p = r - u * N;
+        const UnitDesc& d = D[u];
+        if (d.num_parts == 0) continue;      /* no sources: untouched */
+        reduce_row<FP, TL>(N, p, H->num_tiles, d.num_parts, H->use_casa,
+                d.partial, d.cmax, d.st, d.uv_min, d.uv_max, d.offset_out,
+                (Cplx<FP>*) d.vis);
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Host side.                                                             */
 
 struct Workspace
 {
@@ -562,6 +878,8 @@ void* ws_get(Workspace* w, size_t bytes, hipError_t* err)
     return w->ptr;
 }
 
+inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
 Workspace g_ws[64][2];   /* per device: [0] sources/stations, [1] partials */
 
 /* Optional HIP-event timing of the stages of the last call (per device):
@@ -583,15 +901,25 @@ inline void mark(int dev, int i, hipStream_t stream)
     (void) hipEventRecord(g_ev[dev][i], stream);
 }
 
+template <class TL>
+hipError_t launch_product(int mode, dim3 grid, hipStream_t stream,
+        const FactArgs& A)
+{
+    if (mode == 1)
+        hipLaunchKernelGGL((fact_xcorr_kernel<__bf16, TL>), grid,
+                dim3(THREADS), 0, stream, A);
+    else
+        hipLaunchKernelGGL((fact_xcorr_kernel<_Float16, TL>), grid,
+                dim3(THREADS), 0, stream, A);
+    return hipGetLastError();
+}
+
 } /* namespace */
 
-/* Number of tiles for N stations (host). */
+/* Number of tiles for N stations with the default tile shape (host). */
 extern "C" int oskar_amd_fact_num_tiles(int N)
 {
-    const int np = (N + TP - 1) / TP, nq = (N + TQ - 1) / TQ;
-    int t = 0;
-    for (int i = 0; i < np; ++i) t += tiles_in_row(i, N, nq);
-    return t;
+    return num_tiles(N, Tile0::TP, Tile0::TQ);
 }
 
 extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
@@ -602,6 +930,7 @@ extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
         double uv_min_lambda, double uv_max_lambda, int offset_out,
         void* vis, int src_per_part, void* stream_v)
 {
+    using TL = Tile0;
     const hipStream_t stream = (hipStream_t) stream_v;
     const int N = num_stations, S = num_sources;
     if (N < 2 || S <= 0) return 0;
@@ -612,18 +941,18 @@ extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
     if (src_per_part <= 0) src_per_part = 8192;
     src_per_part = ((src_per_part + KSRC - 1) / KSRC) * KSRC;
     const int parts = (S + src_per_part - 1) / src_per_part;
-    const int tiles = oskar_amd_fact_num_tiles(N);
+    const int tiles = num_tiles(N, TL::TP, TL::TQ);
 
     /* Workspace 0: [cmax (256 B)][stations 3N doubles][src S x 24 B]
      * [cmat S x 16 B]; workspace 1: partials. */
     const size_t off_st = 256;
-    const size_t off_src = off_st + ((3 * (size_t) N * 8 + 255) / 256) * 256;
-    const size_t off_c = off_src + ((sizeof(Src) * (size_t) S + 255) / 256)
-            * 256;
+    const size_t off_src = off_st + align256(3 * (size_t) N * 8);
+    const size_t off_c = off_src + align256(sizeof(Src) * (size_t) S);
     const size_t bytes0 = off_c + 16 * (size_t) S;
     char* w0 = (char*) ws_get(&g_ws[dev][0], bytes0, &err);
     if (err != hipSuccess) return (int) err;
-    const size_t bytes1 = (size_t) parts * tiles * 2 * TP * TW * sizeof(float);
+    const size_t bytes1 = (size_t) parts * tiles * 2 * TL::TP * TL::TW *
+            sizeof(float);
     float* partial = (float*) ws_get(&g_ws[dev][1], bytes1, &err);
     if (err != hipSuccess) return (int) err;
     unsigned int* cmax = (unsigned int*) w0;
@@ -672,27 +1001,19 @@ extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
     A.cmax = cmax;
     A.partial = partial;
     mark(dev, 1, stream);
-    if (mode == 1)
-        hipLaunchKernelGGL(fact_xcorr_kernel<__bf16>, dim3(tiles, parts),
-                dim3(THREADS), 0, stream, A);
-    else
-        hipLaunchKernelGGL(fact_xcorr_kernel<_Float16>, dim3(tiles, parts),
-                dim3(THREADS), 0, stream, A);
-    err = hipGetLastError();
+    err = launch_product<TL>(mode, dim3(tiles, parts), stream, A);
     if (err != hipSuccess) return (int) err;
 
     const double uv_min = uv_min_lambda, uv_max = uv_max_lambda;
     mark(dev, 2, stream);
     if (precision == OSKAR_DOUBLE)
-        hipLaunchKernelGGL(fact_reduce_kernel<double>,
-                dim3((N + 255) / 256, N), dim3(256), 0, stream, N, tiles,
-                parts, use_casa, partial, cmax, st, uv_min, uv_max,
-                offset_out, (Cplx<double>*) vis);
+        hipLaunchKernelGGL((fact_reduce_kernel<double, TL>), dim3(N),
+                dim3(256), 0, stream, N, tiles, parts, use_casa, partial,
+                cmax, st, uv_min, uv_max, offset_out, (Cplx<double>*) vis);
     else
-        hipLaunchKernelGGL(fact_reduce_kernel<float>,
-                dim3((N + 255) / 256, N), dim3(256), 0, stream, N, tiles,
-                parts, use_casa, partial, cmax, st, uv_min, uv_max,
-                offset_out, (Cplx<float>*) vis);
+        hipLaunchKernelGGL((fact_reduce_kernel<float, TL>), dim3(N),
+                dim3(256), 0, stream, N, tiles, parts, use_casa, partial,
+                cmax, st, uv_min, uv_max, offset_out, (Cplx<float>*) vis);
     mark(dev, 3, stream);
     return (int) hipGetLastError();
 }
@@ -715,4 +1036,427 @@ extern "C" int oskar_amd_fact_last_times(double* ms)
         ms[k] = (double) t;
     }
     return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Batch object.                                                          */
+
+namespace {
+
+struct BatchConfig
+{
+    int shape = 0;               /* index into SHAPE_TP / SHAPE_TQ */
+    int src_per_part = 8192;
+    int workgroups = 0;          /* persistent grid; 0 = CUs x occupancy */
+    bool operator==(const BatchConfig& o) const
+    {
+        return shape == o.shape && src_per_part == o.src_per_part &&
+                workgroups == o.workgroups;
+    }
+};
+
+struct Batch
+{
+    int precision = 0, mode = 0, N = 0, max_units = 0;
+    int dev = 0, num_cus = 0, use_graph = 1;
+    BatchConfig cfg;
+    std::vector<UnitDesc> units;         /* host copy, inputs only */
+    std::vector<char> unit_set;
+    BatchHeader* h_hdr = nullptr;        /* pinned staging */
+    UnitDesc* h_desc = nullptr;
+    BatchHeader* d_hdr = nullptr;        /* device */
+    UnitDesc* d_desc = nullptr;
+    int* d_counter = nullptr;
+    Workspace ws;
+    hipEvent_t ev_upload = nullptr, ev_start = nullptr, ev_end = nullptr;
+    bool upload_pending = false, timed = false;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+    BatchConfig exec_cfg;
+};
+
+int occupancy(int shape)
+{
+    switch (shape)
+    {
+    case 1: return Tile1::OCC;
+    case 2: return Tile2::OCC;
+    default: return Tile0::OCC;
+    }
+}
+
+template <typename FP, class TL>
+hipError_t batch_launch_t(Batch* b, hipStream_t s)
+{
+    const int wgs = b->cfg.workgroups > 0 ? b->cfg.workgroups :
+            b->num_cus * TL::OCC;
+    const int rows_grid = std::min(4096, b->max_units * b->N);
+    hipLaunchKernelGGL(fact_batch_stations_kernel<FP>, dim3(1024),
+            dim3(256), 0, s, b->d_hdr, b->d_desc, b->d_counter);
+    hipLaunchKernelGGL(fact_batch_prepare_kernel<FP>, dim3(4096), dim3(256),
+            0, s, b->d_hdr, b->d_desc);
+    if (b->mode == 1)
+        hipLaunchKernelGGL((fact_batch_kernel<__bf16, TL>), dim3(wgs),
+                dim3(THREADS), 0, s, b->d_hdr, b->d_desc, b->d_counter);
+    else
+        hipLaunchKernelGGL((fact_batch_kernel<_Float16, TL>), dim3(wgs),
+                dim3(THREADS), 0, s, b->d_hdr, b->d_desc, b->d_counter);
+    hipLaunchKernelGGL((fact_batch_reduce_kernel<FP, TL>), dim3(rows_grid),
+            dim3(256), 0, s, b->d_hdr, b->d_desc);
+    return hipGetLastError();
+}
+
+template <typename FP>
+hipError_t batch_launch_fp(Batch* b, hipStream_t s)
+{
+    switch (b->cfg.shape)
+    {
+    case 1: return batch_launch_t<FP, Tile1>(b, s);
+    case 2: return batch_launch_t<FP, Tile2>(b, s);
+    default: return batch_launch_t<FP, Tile0>(b, s);
+    }
+}
+
+hipError_t batch_launch(Batch* b, hipStream_t s)
+{
+    return b->precision == OSKAR_DOUBLE ? batch_launch_fp<double>(b, s) :
+            batch_launch_fp<float>(b, s);
+}
+
+/* Fills the descriptors of units [0, nu), sizes the workspace and uploads
+ * header and descriptors on stream s.  vis_override (if not NULL) replaces
+ * every unit's output with consecutive blocks of num_baselines values. */
+hipError_t batch_upload(Batch* b, int nu, int use_casa,
+        const double* uv_min, const double* uv_max, void* vis_override,
+        hipStream_t s)
+{
+    const int N = b->N, TP = SHAPE_TP[b->cfg.shape],
+            TQ = SHAPE_TQ[b->cfg.shape];
+    const int tiles = num_tiles(N, TP, TQ);
+    const int spp = b->cfg.src_per_part;
+    const size_t tile_bytes = 2 * (size_t) TP * 4 * TQ * sizeof(float);
+    const size_t nb = (size_t) N * (N - 1) / 2;
+    const size_t vis_elem = (b->precision == OSKAR_DOUBLE ? 64 : 32);
+
+    /* Workspace layout per unit: [cmax 256 B][stations][src][cmat][partial]. */
+    size_t bytes = 0;
+    long long items = 0, prep = 0;
+    std::vector<size_t> off(nu);
+    for (int u = 0; u < nu; ++u)
+    {
+        const int S = b->units[u].num_sources;
+        const int parts = (S + spp - 1) / spp;
+        off[u] = bytes;
+        bytes += 256 + align256(3 * (size_t) N * 8) +
+                align256(sizeof(Src) * (size_t) S) + align256(16 * (size_t) S)
+                + (size_t) parts * tiles * tile_bytes;
+        items += (long long) parts * tiles;
+        prep += (S + 255) / 256 * 256;
+    }
+    if (items > INT32_MAX || prep > INT32_MAX) return hipErrorInvalidValue;
+    hipError_t err = hipSuccess;
+    if (b->ws.size < bytes)
+    {
+        /* The previous batch may still read the old workspace. */
+        err = hipDeviceSynchronize();
+        if (err != hipSuccess) return err;
+    }
+    char* base = (char*) ws_get(&b->ws, bytes, &err);
+    if (err != hipSuccess) return err;
+
+    /* The previous upload must have left the staging buffers. */
+    if (b->upload_pending)
+    {
+        err = hipEventSynchronize(b->ev_upload);
+        if (err != hipSuccess) return err;
+        b->upload_pending = false;
+    }
+    BatchHeader* h = b->h_hdr;
+    memset(h, 0, sizeof(*h));
+    h->num_units = nu;
+    h->num_stations = N;
+    h->num_tiles = tiles;
+    h->src_per_part = spp;
+    h->total_items = (int) items;
+    h->prep_total = (int) prep;
+    h->use_casa = use_casa;
+    int item_start = 0, prep_start = 0;
+    for (int u = 0; u < nu; ++u)
+    {
+        UnitDesc d = b->units[u];
+        const int S = d.num_sources;
+        const int parts = (S + spp - 1) / spp;
+        char* w = base + off[u];
+        d.cmax = (unsigned int*) w;
+        w += 256;
+        d.st = (double*) w;
+        w += align256(3 * (size_t) N * 8);
+        d.src = (Src*) w;
+        w += align256(sizeof(Src) * (size_t) S);
+        d.cmat = (float4*) w;
+        w += align256(16 * (size_t) S);
+        d.partial = (float*) w;
+        d.num_parts = parts;
+        d.item_start = item_start;
+        d.prep_start = prep_start;
+        item_start += parts * tiles;
+        prep_start += (S + 255) / 256 * 256;
+        d.uv_min = uv_min ? uv_min[u] : 0.0;
+        d.uv_max = uv_max ? uv_max[u] : 1e300;
+        if (vis_override)
+        {
+            d.vis = (char*) vis_override + (size_t) u * nb * vis_elem;
+            d.offset_out = 0;
+        }
+        b->h_desc[u] = d;
+    }
+    err = hipMemcpyAsync(b->d_hdr, h, sizeof(BatchHeader),
+            hipMemcpyHostToDevice, s);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(b->d_desc, b->h_desc, nu * sizeof(UnitDesc),
+                hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) err = hipEventRecord(b->ev_upload, s);
+    if (err == hipSuccess) b->upload_pending = true;
+    return err;
+}
+
+/* Runs the four kernels of a batch on stream s: replays the captured graph
+ * of the current configuration (capturing it first if needed), or launches
+ * directly when graphs are off. */
+hipError_t batch_execute(Batch* b, hipStream_t s)
+{
+    hipError_t err;
+    if (!b->use_graph) return batch_launch(b, s);
+    if (!b->exec || !(b->exec_cfg == b->cfg))
+    {
+        if (b->exec)
+        {
+            err = hipStreamSynchronize(s);
+            if (err != hipSuccess) return err;
+            (void) hipGraphExecDestroy(b->exec);
+            b->exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        err = hipStreamBeginCapture(b->cap_stream,
+                hipStreamCaptureModeThreadLocal);
+        if (err != hipSuccess) return err;
+        const hipError_t le = batch_launch(b, b->cap_stream);
+        err = hipStreamEndCapture(b->cap_stream, &g);
+        if (le != hipSuccess) { if (g) (void) hipGraphDestroy(g); return le; }
+        if (err != hipSuccess) return err;
+        err = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        (void) hipGraphDestroy(g);
+        if (err != hipSuccess) { b->exec = nullptr; return err; }
+        b->exec_cfg = b->cfg;
+    }
+    return hipGraphLaunch(b->exec, s);
+}
+
+void batch_release(Batch* b)
+{
+    if (b->exec) (void) hipGraphExecDestroy(b->exec);
+    if (b->cap_stream) (void) hipStreamDestroy(b->cap_stream);
+    if (b->ev_upload) (void) hipEventDestroy(b->ev_upload);
+    if (b->ev_start) (void) hipEventDestroy(b->ev_start);
+    if (b->ev_end) (void) hipEventDestroy(b->ev_end);
+    if (b->ws.ptr) (void) hipFree(b->ws.ptr);
+    if (b->d_hdr) (void) hipFree(b->d_hdr);
+    if (b->d_desc) (void) hipFree(b->d_desc);
+    if (b->d_counter) (void) hipFree(b->d_counter);
+    if (b->h_hdr) (void) hipHostFree(b->h_hdr);
+    if (b->h_desc) (void) hipHostFree(b->h_desc);
+    delete b;
+}
+
+} /* namespace */
+
+extern "C" void* oskar_amd_fact_batch_create(int precision, int mode,
+        int num_stations, int max_units, int* err_out)
+{
+    Batch* b = new Batch;
+    b->precision = precision;
+    b->mode = mode;
+    b->N = num_stations;
+    b->max_units = max_units;
+    b->units.assign(max_units, UnitDesc());
+    b->unit_set.assign(max_units, 0);
+    hipError_t err = hipGetDevice(&b->dev);
+    if (err == hipSuccess)
+        err = hipDeviceGetAttribute(&b->num_cus,
+                hipDeviceAttributeMultiprocessorCount, b->dev);
+    if (err == hipSuccess)
+        err = hipHostMalloc((void**) &b->h_hdr, sizeof(BatchHeader), 0);
+    if (err == hipSuccess)
+        err = hipHostMalloc((void**) &b->h_desc,
+                max_units * sizeof(UnitDesc), 0);
+    if (err == hipSuccess)
+        err = hipMalloc((void**) &b->d_hdr, sizeof(BatchHeader));
+    if (err == hipSuccess)
+        err = hipMalloc((void**) &b->d_desc, max_units * sizeof(UnitDesc));
+    if (err == hipSuccess) err = hipMalloc((void**) &b->d_counter, 256);
+    if (err == hipSuccess) err = hipEventCreate(&b->ev_upload);
+    if (err == hipSuccess) err = hipEventCreate(&b->ev_start);
+    if (err == hipSuccess) err = hipEventCreate(&b->ev_end);
+    if (err == hipSuccess)
+        err = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+    if (err != hipSuccess)
+    {
+        batch_release(b);
+        *err_out = (int) err;
+        return nullptr;
+    }
+    *err_out = 0;
+    return b;
+}
+
+extern "C" void oskar_amd_fact_batch_free(void* h)
+{
+    Batch* b = (Batch*) h;
+    if (!b) return;
+    (void) hipDeviceSynchronize();
+    batch_release(b);
+}
+
+extern "C" void oskar_amd_fact_batch_set_unit(void* h, int unit,
+        int num_sources, const void* beam, const void* I, const void* Q,
+        const void* U, const void* V, const void* l, const void* m,
+        const void* n, const void* st_u, const void* st_v, const void* st_w,
+        double frequency_hz, int offset_out, void* vis)
+{
+    Batch* b = (Batch*) h;
+    UnitDesc d;
+    memset(&d, 0, sizeof(d));
+    d.I = I; d.Q = Q; d.U = U; d.V = V;
+    d.l = l; d.m = m; d.n = n;
+    d.beam = beam;
+    d.st_u = st_u; d.st_v = st_v; d.st_w = st_w;
+    d.vis = vis;
+    d.f_over_c = frequency_hz / 299792458.0;
+    d.num_sources = num_sources;
+    d.offset_out = offset_out;
+    b->units[unit] = d;
+    b->unit_set[unit] = 1;
+}
+
+extern "C" int oskar_amd_fact_batch_unit_is_set(const void* h, int unit)
+{
+    return ((const Batch*) h)->unit_set[unit];
+}
+
+extern "C" int oskar_amd_fact_batch_set_config(void* h, int shape,
+        int src_per_part, int workgroups, int use_graph)
+{
+    Batch* b = (Batch*) h;
+    if (shape < 0 || shape >= NUM_SHAPES || src_per_part < KSRC ||
+            workgroups < 0) return (int) hipErrorInvalidValue;
+    b->cfg.shape = shape;
+    b->cfg.src_per_part = (src_per_part + KSRC - 1) / KSRC * KSRC;
+    b->cfg.workgroups = workgroups;
+    b->use_graph = use_graph != 0;
+    return 0;
+}
+
+extern "C" void oskar_amd_fact_batch_get_config(const void* h, int* out)
+{
+    const Batch* b = (const Batch*) h;
+    out[0] = b->cfg.shape;
+    out[1] = b->cfg.src_per_part;
+    out[2] = b->cfg.workgroups > 0 ? b->cfg.workgroups :
+            b->num_cus * occupancy(b->cfg.shape);
+    out[3] = b->use_graph;
+}
+
+extern "C" int oskar_amd_fact_batch_run(void* h, int num_units,
+        int use_casa, const double* uv_min, const double* uv_max,
+        void* stream_v)
+{
+    Batch* b = (Batch*) h;
+    const hipStream_t s = (hipStream_t) stream_v;
+    if (num_units <= 0) return 0;
+    hipError_t err = batch_upload(b, num_units, use_casa, uv_min, uv_max,
+            nullptr, s);
+    if (err == hipSuccess) err = hipEventRecord(b->ev_start, s);
+    if (err == hipSuccess) err = batch_execute(b, s);
+    if (err == hipSuccess) err = hipEventRecord(b->ev_end, s);
+    b->timed = (err == hipSuccess);
+    return (int) err;
+}
+
+/* Elapsed time (ms) of the last run's kernels; waits for them. */
+extern "C" int oskar_amd_fact_batch_last_ms(void* h, double* ms)
+{
+    Batch* b = (Batch*) h;
+    if (!b->timed) return -1;
+    float t = 0.0f;
+    if (hipEventSynchronize(b->ev_end) != hipSuccess ||
+            hipEventElapsedTime(&t, b->ev_start, b->ev_end) != hipSuccess)
+        return -1;
+    *ms = (double) t;
+    return 0;
+}
+
+/* Times every candidate configuration (tile shape x part size x grid) on
+ * the first num_units units, with outputs sent to a scratch buffer (the
+ * callers' visibilities are not touched), and keeps the fastest.  Direct
+ * launches, one warm-up and `reps` timed runs each (median). */
+extern "C" int oskar_amd_fact_batch_autotune(void* h, int num_units,
+        int reps, void* stream_v, double* best_ms)
+{
+    Batch* b = (Batch*) h;
+    const hipStream_t s = (hipStream_t) stream_v;
+    if (num_units <= 0) return (int) hipErrorInvalidValue;
+    if (reps < 1) reps = 1;
+    const size_t nb = (size_t) b->N * (b->N - 1) / 2;
+    const size_t vis_bytes = (size_t) num_units * nb *
+            (b->precision == OSKAR_DOUBLE ? 64 : 32);
+    void* scratch = nullptr;
+    hipError_t err = hipMalloc(&scratch, vis_bytes);
+    if (err != hipSuccess) return (int) err;
+    const BatchConfig saved = b->cfg;
+    const int saved_graph = b->use_graph;
+    b->use_graph = 0;
+    BatchConfig best = saved;
+    double best_t = 1e300;
+    static const int parts[] = {4096, 8192, 16384};
+    for (int shape = 0; shape < NUM_SHAPES && err == hipSuccess; ++shape)
+    {
+        for (int pi = 0; pi < 3 && err == hipSuccess; ++pi)
+        {
+            for (int occ = 1; occ <= occupancy(shape) && err == hipSuccess;
+                    ++occ)
+            {
+                BatchConfig c;
+                c.shape = shape;
+                c.src_per_part = parts[pi];
+                c.workgroups = b->num_cus * occ;
+                b->cfg = c;
+                std::vector<double> t;
+                for (int r = 0; r <= reps && err == hipSuccess; ++r)
+                {
+                    err = batch_upload(b, num_units, 0, nullptr, nullptr,
+                            scratch, s);
+                    if (err == hipSuccess) err = hipEventRecord(b->ev_start, s);
+                    if (err == hipSuccess) err = batch_launch(b, s);
+                    if (err == hipSuccess) err = hipEventRecord(b->ev_end, s);
+                    if (err == hipSuccess) err = hipEventSynchronize(b->ev_end);
+                    float ms = 0.0f;
+                    if (err == hipSuccess)
+                        err = hipEventElapsedTime(&ms, b->ev_start, b->ev_end);
+                    if (err == hipSuccess && r > 0) t.push_back(ms);
+                }
+                if (err != hipSuccess) break;
+                std::sort(t.begin(), t.end());
+                const double med = t[t.size() / 2];
+                if (med < best_t) { best_t = med; best = c; }
+            }
+        }
+    }
+    const hipError_t e2 = hipDeviceSynchronize();
+    (void) hipFree(scratch);
+    b->use_graph = saved_graph;
+    b->cfg = (err == hipSuccess) ? best : saved;
+    b->timed = false;
+    if (err == hipSuccess) err = e2;
+    if (err == hipSuccess && best_ms) *best_ms = best_t;
+    return (int) err;
 }

@@ -1,5 +1,5 @@
 """Python handles over the C-ABI structures (oskar_Mem, oskar_Jones,
-oskar_Telescope, oskar_Station, oskar_StationWork).
+oskar_Telescope, oskar_Station, oskar_StationWork, oskar_FactorisedBatch).
 
 Mirrors the way the reference's own tests drive the C API
 (oskar/correlate/test/Test_cross_correlate.cpp:60-120): create typed
@@ -10,6 +10,7 @@ import ctypes as C
 
 import numpy as np
 
+from . import capi
 from .capi import lib
 
 OSKAR_CPU, OSKAR_GPU = 0, 1
@@ -236,5 +237,75 @@ class StationWork:
             if self.ptr:
                 lib.oskar_station_work_free(self.ptr, C.byref(C.c_int(0)))
                 self.ptr = None
+        except Exception:
+            pass
+
+
+class FactorisedBatch:
+    """oskar_FactorisedBatch: many (channel, time) units in one persistent
+    launch.  Keeps a reference to every unit's Mem objects, which the C side
+    only borrows."""
+
+    def __init__(self, precision, mode, num_stations, max_units):
+        st = C.c_int(0)
+        self.ptr = lib.oskar_factorised_batch_create(
+            precision, mode, num_stations, max_units, C.byref(st))
+        check_status(st, "oskar_factorised_batch_create")
+        self._keep = {}
+
+    def set_unit(self, unit, num_sources, beam, flux, dirs, uvw, freq_hz,
+                 offset_out, vis):
+        """beam: Mem or None; flux: 4 Mem (I, Q, U, V); dirs: 3 Mem (l, m,
+        n); uvw: 3 Mem (metres); vis: Mem (complex matrix)."""
+        st = C.c_int(0)
+        lib.oskar_factorised_batch_set_unit(
+            self.ptr, int(unit), int(num_sources),
+            beam.ptr if beam is not None else None,
+            capi.MEMA(*[x.ptr for x in flux]),
+            capi.MEM3(*[x.ptr for x in dirs]),
+            capi.MEM3(*[x.ptr for x in uvw]), float(freq_hz),
+            int(offset_out), vis.ptr, C.byref(st))
+        check_status(st, "oskar_factorised_batch_set_unit")
+        self._keep[unit] = (beam, tuple(flux), tuple(dirs), tuple(uvw), vis)
+
+    def run(self, num_units, use_casa, tel):
+        st = C.c_int(0)
+        lib.oskar_factorised_batch_run(self.ptr, int(num_units),
+                                       int(use_casa), tel.ptr, C.byref(st))
+        check_status(st, "oskar_factorised_batch_run")
+
+    def autotune(self, num_units):
+        st = C.c_int(0)
+        lib.oskar_factorised_batch_autotune(self.ptr, int(num_units),
+                                            C.byref(st))
+        check_status(st, "oskar_factorised_batch_autotune")
+        return self.config()
+
+    def set_config(self, tile_shape, src_per_part, num_workgroups=0,
+                   use_graph=1):
+        st = C.c_int(0)
+        lib.oskar_factorised_batch_set_config(
+            self.ptr, int(tile_shape), int(src_per_part),
+            int(num_workgroups), int(use_graph), C.byref(st))
+        check_status(st, "oskar_factorised_batch_set_config")
+
+    def config(self):
+        out = (C.c_int * 4)()
+        lib.oskar_factorised_batch_config(self.ptr, out)
+        return dict(tile_shape=out[0], src_per_part=out[1],
+                    workgroups=out[2], use_graph=out[3])
+
+    def last_run_ms(self):
+        return lib.oskar_factorised_batch_last_run_ms(self.ptr)
+
+    def free(self):
+        if self.ptr:
+            lib.oskar_factorised_batch_free(self.ptr, C.byref(C.c_int(0)))
+            self.ptr = None
+            self._keep = {}
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass

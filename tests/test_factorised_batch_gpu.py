@@ -1,0 +1,271 @@
+"""GPU tests of the batched persistent factorised correlator
+(oskar_factorised_batch_*) through the C-ABI.
+
+A batch holds (channel, time) units with uneven source counts (0, 5, 256,
+4097, 9001 and 16387 sources), units of one time sample sharing the same
+source and station buffers, and outputs both in blocks of one visibility
+array (the [time][channel][baseline] layout of a visibility block) and in a
+separate array.  Checks:
+- each unit equals the one-call path bit for bit at the same tile shape and
+  part size (same arithmetic, same FP64 summation order);
+- every tile shape and part size is within the factorised tolerance of the
+  FP64 oracle of the reference's direct algorithm (1.5e-5 relative RMS, as
+  for the one-call path);
+- hipGraph replays, with other GPU work between them, equal direct
+  launches exactly;
+- uv filter in metres (a different wavelength limit per channel), CASA
+  convention and accumulation into existing visibilities;
+- the autotuner leaves the callers' visibilities untouched and caches its
+  choice.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+import pytest
+
+from fact_cases import make_case, oracle_direct, rounded
+from gpu_util import gpu_factorised, prec_code
+from helpers import rel_rms
+from oskar_amd import lib
+from oskar_amd.mem import (OSKAR_COMPLEX, OSKAR_GPU, OSKAR_MATRIX,
+                           FactorisedBatch, Mem, OskarError, Telescope,
+                           check_status)
+
+pytestmark = pytest.mark.gpu
+NS = 150
+NSRC = 16387
+# (sources, frequency in MHz, time sample, beam on)
+UNITS = [(9001, 150.0, 0, True), (5, 50.0, 0, True), (0, 80.0, 1, True),
+         (NSRC, 350.0, 1, False), (256, 100.0, 1, True),
+         (4097, 220.0, 0, True)]
+TOL = 1.5e-5
+_SKIES = []
+_ORACLE = {}
+
+
+def skies():
+    """Two time samples: different station (u, v, w) and catalogues."""
+    if not _SKIES:
+        _SKIES.extend([make_case(101, NS, NSRC), make_case(202, NS, NSRC)])
+    return _SKIES
+
+
+def unit_case(u):
+    S, f, t, beam = UNITS[u]
+    c = skies()[t]
+    sub = {k: (None if c[k] is None else c[k][:S])
+           for k in ("l", "m", "n", "I", "Q", "U", "V")}
+    sub.update(u=c["u"], v=c["v"], w=c["w"])
+    sub["E"] = c["E"][:S] if beam else None
+    return sub, f * 1e6
+
+
+def oracle_unit(prec, u):
+    key = (prec, u)
+    if key not in _ORACLE:
+        sub, f = unit_case(u)
+        if sub["l"].size == 0:
+            _ORACLE[key] = np.zeros(NS * (NS - 1) // 2 * 8)
+        else:
+            _ORACLE[key] = oracle_direct(rounded(sub, prec), f)
+    return _ORACLE[key]
+
+
+def single_call(prec, mode, u):
+    sub, f = unit_case(u)
+    nb = NS * (NS - 1) // 2
+    if sub["l"].size == 0:
+        return np.zeros(nb * 8)
+    E = None if sub["E"] is None else sub["E"].ravel()
+    return gpu_factorised(prec, mode, 0, E, sub["I"], sub["Q"], sub["U"],
+                          sub["V"], sub["l"], sub["m"], sub["n"], sub["u"],
+                          sub["v"], sub["w"], f)
+
+
+class BatchCase:
+    """Device inputs of all units; units of one time sample share Mem
+    objects (and pass their own source counts)."""
+
+    def __init__(self, prec, mode, max_units=None, tel=None):
+        P = prec_code(prec)
+        self.prec, self.mode = prec, mode
+        self.mt = P | OSKAR_COMPLEX | OSKAR_MATRIX
+        self.nb = NS * (NS - 1) // 2
+        self.dev = []
+        for c in skies():
+            self.dev.append(dict(
+                flux=[Mem.from_numpy(c[k], P) for k in "IQUV"],
+                dirs=[Mem.from_numpy(c[k], P) for k in "lmn"],
+                uvw=[Mem.from_numpy(c[k], P) for k in "uvw"],
+                beam=Mem.from_numpy(c["E"].ravel(), self.mt)))
+        self.tel = tel if tel is not None else Telescope(P, OSKAR_GPU, NS)
+        nu = len(UNITS)
+        self.vis = Mem.from_numpy(np.zeros((nu - 1) * self.nb * 8), self.mt)
+        self.vis_own = Mem.from_numpy(np.zeros(self.nb * 8), self.mt)
+        self.batch = FactorisedBatch(P, mode, NS, max_units or nu)
+        for u, (S, f, t, beam) in enumerate(UNITS):
+            d = self.dev[t]
+            last = u == nu - 1
+            self.batch.set_unit(u, S, d["beam"] if beam else None, d["flux"],
+                                d["dirs"], d["uvw"], f * 1e6,
+                                0 if last else u * self.nb,
+                                self.vis_own if last else self.vis)
+
+    def set_vis(self, init):
+        """init: (units, nb * 8) reals."""
+        nu = len(UNITS)
+        self.vis = Mem.from_numpy(init[:nu - 1].ravel(), self.mt)
+        self.vis_own = Mem.from_numpy(init[nu - 1], self.mt)
+        for u, (S, f, t, beam) in enumerate(UNITS):
+            d = self.dev[t]
+            last = u == nu - 1
+            self.batch.set_unit(u, S, d["beam"] if beam else None, d["flux"],
+                                d["dirs"], d["uvw"], f * 1e6,
+                                0 if last else u * self.nb,
+                                self.vis_own if last else self.vis)
+
+    def clear(self):
+        self.vis.clear()
+        self.vis_own.clear()
+
+    def run(self, casa=0, num_units=None):
+        self.batch.run(len(UNITS) if num_units is None else num_units, casa,
+                       self.tel)
+        lib.oskar_device_synchronize(OSKAR_GPU)
+
+    def results(self):
+        a = self.vis.numpy().reshape(len(UNITS) - 1, -1)
+        return [a[u].copy() for u in range(len(UNITS) - 1)] + [
+            self.vis_own.numpy()]
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_batch_equals_one_call_bitwise(hip, prec):
+    print("batch vs one call", prec, flush=True)
+    case = BatchCase(prec, 0)
+    case.batch.set_config(0, 8192, 0, 1)
+    case.run()
+    got = case.results()
+    for u in range(len(UNITS)):
+        want = single_call(prec, 0, u)
+        assert np.array_equal(got[u], want), "unit %d differs" % u
+        if UNITS[u][0] == 0:
+            assert not np.any(got[u])
+
+
+@pytest.mark.parametrize("shape,part,mode", [
+    (0, 4096, 0), (0, 16384, 0), (1, 8192, 0), (1, 4096, 0), (2, 8192, 0),
+    (2, 16384, 0), (0, 8192, 1), (1, 8192, 1), (2, 4096, 1)])
+def test_batch_shapes_and_parts_match_fp64_oracle(hip, shape, part, mode):
+    print("batch shape", shape, "part", part, "mode", mode, flush=True)
+    case = BatchCase("f", mode)
+    case.batch.set_config(shape, part, 0, 1)
+    case.run()
+    got = case.results()
+    for u in range(len(UNITS)):
+        want = oracle_unit("f", u)
+        if UNITS[u][0] == 0:
+            assert not np.any(got[u])
+            continue
+        err = rel_rms(got[u], want)
+        print("  unit %d S=%d rel RMS %.3e" % (u, UNITS[u][0], err),
+              flush=True)
+        assert err < TOL
+
+
+def test_batch_graph_replays_equal_direct_launch(hip):
+    print("batch graph replays", flush=True)
+    case = BatchCase("f", 0)
+    case.batch.set_config(1, 4096, 0, 0)          # direct launches
+    case.run()
+    ref = case.results()
+    case.batch.set_config(1, 4096, 0, 1)          # captured graph
+    for rep in range(3):
+        case.clear()
+        # Other work on the GPU between replays.
+        single_call("f", 0, 5)
+        case.run()
+        got = case.results()
+        for u in range(len(UNITS)):
+            assert np.array_equal(got[u], ref[u]), (rep, u)
+    # A replay with fewer units leaves the others' outputs alone.
+    case.clear()
+    case.run(num_units=2)
+    got = case.results()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    for u in range(2, len(UNITS)):
+        assert not np.any(got[u])
+
+
+def test_batch_uv_filter_casa_and_accumulation(hip):
+    print("batch uv filter casa accumulate", flush=True)
+    tel = Telescope(prec_code("d"), OSKAR_GPU, NS)
+    s = C.c_int(0)
+    lib.oskar_telescope_set_uv_filter(tel.ptr, 2000.0, 15000.0, b"Metres",
+                                      C.byref(s))
+    check_status(s, "set_uv_filter")
+    case = BatchCase("d", 0, tel=tel)
+    rng = np.random.default_rng(5)
+    init = rng.normal(size=(len(UNITS), case.nb * 8))
+    case.set_vis(init)
+    case.run(casa=1)
+    got = case.results()
+    for u in range(len(UNITS)):
+        sub, f = unit_case(u)
+        if sub["l"].size == 0:
+            assert np.array_equal(got[u], init[u])
+            continue
+        want = oracle_direct(sub, f, casa=1, vis=init[u].copy(),
+                             uv_min=2000.0, uv_max=15000.0)
+        untouched = want == init[u]
+        assert np.array_equal(got[u][untouched], init[u][untouched])
+        assert rel_rms(got[u] - init[u], want - init[u]) < TOL
+
+
+def test_batch_autotune_leaves_outputs_and_caches(hip):
+    print("batch autotune", flush=True)
+    case = BatchCase("f", 0)
+    rng = np.random.default_rng(7)
+    init = rng.normal(size=(len(UNITS), case.nb * 8)).astype(np.float32)
+    case.set_vis(init.astype(np.float64))
+    t0 = time.perf_counter()
+    cfg = case.batch.autotune(len(UNITS))
+    t_first = time.perf_counter() - t0
+    print("  tuned", cfg, "in %.2f s" % t_first, flush=True)
+    assert cfg["tile_shape"] in (0, 1, 2)
+    assert cfg["src_per_part"] in (4096, 8192, 16384)
+    got = case.results()
+    for u in range(len(UNITS)):
+        assert np.array_equal(got[u], init[u].astype(np.float64))
+    # The tuned configuration computes the right answer.
+    case.clear()
+    case.run()
+    got = case.results()
+    for u in range(len(UNITS)):
+        if UNITS[u][0]:
+            assert rel_rms(got[u], oracle_unit("f", u)) < TOL
+    # A second batch of the same kind reuses the choice without timing.
+    other = BatchCase("f", 0)
+    t0 = time.perf_counter()
+    cfg2 = other.batch.autotune(len(UNITS))
+    assert cfg2 == cfg
+    assert time.perf_counter() - t0 < t_first
+
+
+def test_batch_argument_errors(hip):
+    case = BatchCase("f", 0, max_units=len(UNITS))
+    with pytest.raises(OskarError):
+        case.batch.run(len(UNITS) + 1, 0, case.tel)
+    with pytest.raises(OskarError):
+        case.batch.set_config(3, 8192)
+    d = case.dev[0]
+    with pytest.raises(OskarError):
+        case.batch.set_unit(len(UNITS), 10, None, d["flux"], d["dirs"],
+                            d["uvw"], 1e8, 0, case.vis_own)
+    with pytest.raises(OskarError):       # output too short
+        case.batch.set_unit(0, 10, None, d["flux"], d["dirs"], d["uvw"],
+                            1e8, case.nb, case.vis_own)
+    fresh = FactorisedBatch(prec_code("f"), 0, NS, 4)
+    with pytest.raises(OskarError):       # unit 0 never set
+        fresh.run(1, 0, case.tel)
