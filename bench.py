@@ -83,6 +83,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--error-baselines", type=int, default=48)
+    ap.add_argument("--batch-units", type=int, default=16,
+                    help="(channel, time) units of the batched run; 0 = off")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--batch-error-baselines", type=int, default=8)
+    ap.add_argument("--no-batch-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -271,6 +276,11 @@ def main():
         log("rel RMS vs FP64: factorised %.3e plain %s" % (
             err_fact, "%.3e" % err_plain if err_plain is not None else "-"))
 
+    # --- batched (channel, time) units in one persistent launch ---------
+    batch = None
+    if args.batch_units > 0:
+        batch = bench_batch(args, st, station, work, mode)
+
     # --- CPU baseline: the reference's own correlator ------------------
     cpu = None
     if not args.no_cpu:
@@ -353,8 +363,167 @@ def main():
                      "beam": round(1e3 * t_beam / args.steps, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_batch(args, st, station, work, mode):
+    """A representative subset of the 384-channel (50-350 MHz) x 10-sample
+    observation at 512 stations: args.batch_units units spread over 4 time
+    samples and evenly spaced channels.  Each time sample has its own
+    above-horizon source count (200k at the first sample to 1M at the last),
+    its own sky draw and station (u, v, w) (rotated by the Earth over 60 s
+    per sample); each unit has its own station beam (exact FP32 DFT at its
+    frequency) and fluxes (spectral index -0.7).  All units run in one
+    persistent launch replayed from a hipGraph, with the tile shape, part
+    size and grid chosen by the autotuner.  Error: relative RMS against the
+    FP64 oracle on a sample of baselines in every unit."""
+    from oskar_amd import lib
+    from oskar_amd.mem import (OSKAR_COMPLEX, OSKAR_GPU, OSKAR_MATRIX,
+                               OSKAR_SINGLE, FactorisedBatch, Mem, Telescope,
+                               check_status)
+    F32 = OSKAR_SINGLE
+    CM = F32 | OSKAR_COMPLEX | OSKAR_MATRIX
+    N = st["u"].size
+    NB = N * (N - 1) // 2
+    T, NCH = 10, 384
+    nu = args.batch_units
+    nt = min(4, nu)
+    times = np.linspace(0, T - 1, nt).round().astype(int)
+    per_t = [nu // nt + (1 if i < nu % nt else 0) for i in range(nt)]
+    counts = {t: int(200_000 + 800_000 * t / (T - 1)) for t in range(T)}
+    freqs = 50e6 + 300e6 * np.arange(NCH) / (NCH - 1)
+    rng = np.random.default_rng(77)
+    samples = {}
+    units = []
+    for i, t in enumerate(times):
+        S = counts[t]
+        z = rng.uniform(0.0, 1.0, S)
+        z[z == 0.0] = 1e-7
+        ph = rng.uniform(0, 2 * np.pi, S)
+        s = np.sqrt(1.0 - z * z)
+        sky = dict(l=s * np.sin(ph), m=s * np.cos(ph), n=z,
+                   I=power_law(rng, S))
+        ang = 2 * np.pi * 60.0 * t / 86164.0905
+        ca, sa = np.cos(ang), np.sin(ang)
+        uvw = dict(u=ca * st["u"] - sa * st["v"], v=sa * st["u"] + ca * st["v"],
+                   w=st["w"])
+        samples[t] = dict(
+            sky=sky, uvw=uvw,
+            dirs=[Mem.from_numpy(sky[k], F32) for k in ("l", "m", "n")],
+            zero=Mem.from_numpy(np.zeros(S), F32),
+            uvw_m=[Mem.from_numpy(uvw[k], F32) for k in ("u", "v", "w")])
+        for c in np.linspace(0, NCH - 1, per_t[i]).round().astype(int):
+            units.append((t, int(c)))
+    log("batch: %d units, sources per sample %s" % (
+        len(units), [counts[t] for t in times]))
+    vis = Mem(CM, OSKAR_GPU, len(units) * NB)
+    tel = Telescope(F32, OSKAR_GPU, N)
+    b = FactorisedBatch(F32, mode, N, len(units))
+    beams, fluxes = [], []
+    status = C.c_int(0)
+    total_evals = 0.0
+
+    def eval_beams():
+        for k, (t, c) in enumerate(units):
+            d = samples[t]["dirs"]
+            lib.oskar_evaluate_station_beam_aperture_array(
+                station.ptr, work.ptr, counts[t], d[0].ptr, d[1].ptr,
+                d[2].ptr, 0, 0.0, float(freqs[c]), beams[k].ptr,
+                C.byref(status))
+
+    for k, (t, c) in enumerate(units):
+        smp = samples[t]
+        S = counts[t]
+        beams.append(Mem(CM, OSKAR_GPU, S))
+        fluxes.append(Mem.from_numpy(
+            smp["sky"]["I"] * (freqs[c] / 150e6) ** -0.7, F32))
+        flux = [fluxes[k], smp["zero"], smp["zero"], smp["zero"]]
+        b.set_unit(k, S, beams[k], flux, smp["dirs"], smp["uvw_m"],
+                   float(freqs[c]), k * NB, vis)
+        total_evals += float(NB) * S
+    eval_beams()
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    check_status(status, "batch beams")
+    t0 = time.time()
+    if not args.no_batch_tune:
+        cfg = b.autotune(len(units))
+        log("batch autotune: %s in %.1f s" % (cfg, time.time() - t0))
+    cfg = b.config()
+    # Warm-up (captures the graph), then timed replays.
+    b.run(len(units), 0, tel)
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    log("batch: graph captured")
+    kernel_ms = []
+    t0 = time.time()
+    for i in range(args.batch_steps):
+        b.run(len(units), 0, tel)
+        kernel_ms.append(b.last_run_ms())
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    wall = (time.time() - t0) / args.batch_steps
+    s = C.c_int(0)
+    lib.oskar_device_check_error_hip(C.byref(s))
+    check_status(s, "batch")
+    kms = float(np.median(kernel_ms))
+    log("batch: %d runs, kernels %.2f ms (median), wall %.2f ms per run" % (
+        args.batch_steps, kms, 1e3 * wall))
+    # Beams + batch, as one pipeline step over all units.
+    t0 = time.time()
+    for i in range(args.batch_steps):
+        eval_beams()
+        b.run(len(units), 0, tel)
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    wall_pipe = (time.time() - t0) / args.batch_steps
+    check_status(status, "batch beams")
+    log("batch: beams + batch %.2f ms per run" % (1e3 * wall_pipe))
+
+    # Error of every unit on a sample of baselines against FP64.
+    err = []
+    if not args.no_error:
+        import oracle
+        vis.clear()
+        b.run(len(units), 0, tel)
+        lib.oskar_device_synchronize(OSKAR_GPU)
+        v_all = vis.numpy().reshape(len(units), NB, 8)
+        f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+        nbs = args.batch_error_baselines
+        t0 = time.time()
+        for k, (t, c) in enumerate(units):
+            smp = samples[t]
+            sky, uvw = smp["sky"], smp["uvw"]
+            p = rng.integers(1, N, nbs)
+            q = np.array([rng.integers(0, a) for a in p])
+            bl = q * (N - 1) - (q - 1) * q // 2 + p - q - 1
+            ref = oracle.factorised_subset(
+                f32(sky["l"]), f32(sky["m"]), f32(sky["n"]),
+                f32(fluxes[k].numpy()), beams[k].numpy().astype(np.float64),
+                f32(uvw["u"]), f32(uvw["v"]), f32(uvw["w"]),
+                2 * np.pi * freqs[c] / C0, p.astype(np.int32),
+                q.astype(np.int32)).reshape(-1, 8)
+            got = v_all[k][bl]
+            err.append(float(np.sqrt(np.sum((got - ref) ** 2) /
+                                     np.sum(ref ** 2))))
+        log("batch: FP64 oracle on %d baselines per unit: %.1f s; max rel "
+            "RMS %.3e" % (nbs, time.time() - t0, max(err)))
+    return {
+        "workload": "%d of 3840 (channel, time) units of 384 ch (50-350 MHz) "
+                    "x 10 samples, 512 stations, 200k-1M sources per sample"
+                    % len(units),
+        "units": [{"time": int(t), "channel": int(c),
+                   "freq_hz": float(freqs[c]), "sources": counts[t]}
+                  for t, c in units],
+        "evals": total_evals,
+        "kernel_ms": round(kms, 3),
+        "evals_per_s": total_evals / (kms * 1e-3),
+        "wall_ms_per_run": round(1e3 * wall, 3),
+        "pipeline_ms_per_run_with_beams": round(1e3 * wall_pipe, 3),
+        "pipeline_evals_per_s_with_beams": total_evals / wall_pipe,
+        "config": cfg,
+        "autotuned": not args.no_batch_tune,
+        "rel_rms_vs_fp64_per_unit": [float("%.3e" % e) for e in err],
+        "rel_rms_vs_fp64_max": max(err) if err else None,
+    }
 
 
 if __name__ == "__main__":
