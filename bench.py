@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-error-baselines", type=int, default=8)
     ap.add_argument("--no-batch-tune", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=20000,
+                    help="training steps of the beam surrogate; 0 = off")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -245,6 +247,7 @@ def main():
 
     # --- error against FP64 ------------------------------------------
     err_fact = err_plain = None
+    ref = bl = None
     if not args.no_error:
         import oracle
         nb_s = min(args.error_baselines, NB)
@@ -275,6 +278,13 @@ def main():
             err_plain = rr(vis_plain)
         log("rel RMS vs FP64: factorised %.3e plain %s" % (
             err_fact, "%.3e" % err_plain if err_plain is not None else "-"))
+
+    # --- learned beam surrogate feeding the factorised correlator -------
+    surrogate = None
+    if args.surrogate_steps > 0:
+        surrogate = bench_surrogate(args, st, sky, station, work, mode, tel,
+                                    flux, dirs, uvw, beam, ref, bl, timed,
+                                    evals)
 
     # --- batched (channel, time) units in one persistent launch ---------
     batch = None
@@ -364,6 +374,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batch": batch,
+        "surrogate": surrogate,
     }
     print(json.dumps(line), flush=True)
 
@@ -523,6 +534,143 @@ def bench_batch(args, st, station, work, mode):
         "autotuned": not args.no_batch_tune,
         "rel_rms_vs_fp64_per_unit": [float("%.3e" % e) for e in err],
         "rel_rms_vs_fp64_max": max(err) if err else None,
+    }
+
+
+
+def bench_surrogate(args, st, sky, station, work, mode, tel, flux, dirs, uvw,
+                    beam_exact, ref, bl, timed, evals):
+    """The learned station-beam surrogate of the 256-dipole station over
+    50-350 MHz: trained on the GPU on 10M samples of the exact beam (200
+    frequencies x 50,000 directions), then used in place of the exact beam
+    for the 1M-source workload at 150 MHz (with the exact beam where the
+    surrogate is below 1e-3 of the peak), feeding the factorised correlator.
+    Reports training time, the beam error over the sources, the fused
+    inference time against the exact DFT at 256 and at 4096 elements (the
+    surrogate's cost does not depend on the element count), and the
+    visibility error of the surrogate pipeline against FP64 with the exact
+    beam."""
+    from oskar_amd import capi, lib
+    from oskar_amd.mem import (OSKAR_COMPLEX, OSKAR_GPU, OSKAR_MATRIX,
+                               OSKAR_SINGLE, BeamSurrogate, Mem, Station,
+                               StationWork, beam_training_set, check_status)
+    F32 = OSKAR_SINGLE
+    CM = F32 | OSKAR_COMPLEX | OSKAR_MATRIX
+    N = st["u"].size
+    NB = N * (N - 1) // 2
+    S = sky["l"].size
+    f = args.freq
+    l0, m0 = 0.0, 0.5            # beam at azimuth 0, elevation 60 deg
+    t0 = time.time()
+    tl, tm, tf, tb = beam_training_set(station, work, 200, 50000, 50e6,
+                                       350e6, l0, m0, seed=11)
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    t_set = time.time() - t0
+    log("surrogate: %d exact-beam samples in %.2f s" % (len(tl), t_set))
+    sur = BeamSurrogate(50e6, 350e6, l0, m0, 19.0, seed=5)
+    t0 = time.time()
+    hist = sur.train(len(tl), tl, tm, tf, tb, 8192, args.surrogate_steps,
+                     2e-3, 2e-5, seed=6)
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    t_train = time.time() - t0
+    log("surrogate: %d steps in %.2f s, loss %.3e -> %.3e" % (
+        args.surrogate_steps, t_train, hist[:10].mean(), hist[-50:].mean()))
+    del tl, tm, tf, tb
+    status = C.c_int(0)
+    bs = Mem(CM, OSKAR_GPU, S)
+
+    def step_sur():
+        sur.evaluate(S, dirs[0], dirs[1], f, bs)
+
+    def step_exact():
+        lib.oskar_evaluate_station_beam_aperture_array(
+            station.ptr, work.ptr, S, dirs[0].ptr, dirs[1].ptr, dirs[2].ptr,
+            0, 0.0, f, beam_exact.ptr, C.byref(status))
+
+    reps = 20
+    t_sur = timed(step_sur, reps, 2, "surrogate inference") / reps
+    t_ex = timed(step_exact, reps, 2, "exact beam 256") / reps
+    # A 4096-element station of the same density (radius 76 m).
+    rng = np.random.default_rng(9)
+    ne = 4096
+    re = 76.0 * np.sqrt(rng.uniform(size=ne))
+    te = rng.uniform(0, 2 * np.pi, ne)
+    el = np.stack([re * np.cos(te), re * np.sin(te), np.zeros(ne)])
+    big = Station(F32, OSKAR_GPU, ne)
+    big.set_coords(el, el)
+    lib.oskar_station_set_phase_centre(big.ptr, 4, 0.0, np.radians(60.0))
+    wbig = StationWork(F32, OSKAR_GPU)
+    bbig = Mem(CM, OSKAR_GPU, S)
+
+    def step_big():
+        lib.oskar_evaluate_station_beam_aperture_array(
+            big.ptr, wbig.ptr, S, dirs[0].ptr, dirs[1].ptr, dirs[2].ptr, 0,
+            0.0, f, bbig.ptr, C.byref(status))
+
+    t_big = timed(step_big, 5, 1, "exact beam 4096") / 5
+    del bbig, big, wbig
+    # Beam error over the sources (exact beam at 150 MHz as truth).
+    step_exact()
+    step_sur()
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    be = beam_exact.numpy().astype(np.float64)
+    bsur = bs.numpy().astype(np.float64)
+    beam_err = float(np.sqrt(np.sum((bsur - be) ** 2) / np.sum(be ** 2)))
+    # With the exact-beam fallback below 1e-3 of the peak.
+    nfb = sur.evaluate_with_fallback(station, work, S, dirs[0], dirs[1],
+                                     dirs[2], f, 1e-3, bs)
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    log("surrogate: inference %.3f ms, exact 256 el %.3f ms, exact 4096 el "
+        "%.3f ms; beam rel RMS %.3e; %d exact fallbacks" % (
+            1e3 * t_sur, 1e3 * t_ex, 1e3 * t_big, beam_err, nfb))
+    # Surrogate beam -> factorised correlator.
+    vis = Mem(CM, OSKAR_GPU, NB)
+
+    def step_pipe():
+        step_sur()
+        lib.oskar_cross_correlate_factorised(
+            0, S, bs.ptr, capi.MEMA(*[x.ptr for x in flux]),
+            capi.MEM3(*[x.ptr for x in dirs]), tel.ptr,
+            capi.MEM3(*[x.ptr for x in uvw]), f, mode, 0, vis.ptr,
+            C.byref(status))
+
+    t_pipe = timed(step_pipe, args.steps, 1, "surrogate pipeline") / \
+        args.steps
+    vis_err = None
+    if ref is not None:
+        vis.clear()
+        sur.evaluate_with_fallback(station, work, S, dirs[0], dirs[1],
+                                   dirs[2], f, 1e-3, bs)
+        step = lambda: lib.oskar_cross_correlate_factorised(
+            0, S, bs.ptr, capi.MEMA(*[x.ptr for x in flux]),
+            capi.MEM3(*[x.ptr for x in dirs]), tel.ptr,
+            capi.MEM3(*[x.ptr for x in uvw]), f, mode, 0, vis.ptr,
+            C.byref(status))
+        step()
+        lib.oskar_device_synchronize(OSKAR_GPU)
+        got = vis.numpy().reshape(-1, 8)[bl]
+        vis_err = float(np.sqrt(np.sum((got - ref) ** 2) / np.sum(ref ** 2)))
+        log("surrogate pipeline: vis rel RMS vs FP64 (exact beam) %.3e" %
+            vis_err)
+    check_status(status, "surrogate")
+    return {
+        "workload": "256-dipole station, 50-350 MHz; 10M exact-beam "
+                    "training samples (200 freq x 50k directions); "
+                    "inference for the 1M sources at 150 MHz",
+        "training_samples": 200 * 50000,
+        "training_set_s": round(t_set, 3),
+        "train_steps": args.surrogate_steps, "batch": 8192,
+        "train_s": round(t_train, 3),
+        "loss_first": float(hist[:10].mean()),
+        "loss_last": float(hist[-50:].mean()),
+        "beam_rel_rms_vs_exact": beam_err,
+        "inference_ms_1M": round(1e3 * t_sur, 4),
+        "exact_beam_ms_1M_256el": round(1e3 * t_ex, 4),
+        "exact_beam_ms_1M_4096el": round(1e3 * t_big, 4),
+        "exact_fallbacks_below_1e-3_peak": nfb,
+        "pipeline_ms": round(1e3 * t_pipe, 3),
+        "pipeline_evals_per_s": evals / t_pipe,
+        "vis_rel_rms_vs_fp64_exact_beam": vis_err,
     }
 
 

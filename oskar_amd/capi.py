@@ -13,6 +13,7 @@ P = C.c_void_p
 I = C.c_int
 D = C.c_double
 F = C.c_float
+FP = C.POINTER(C.c_float)
 IP = C.POINTER(C.c_int)
 DP = C.POINTER(C.c_double)
 SZ = C.c_size_t
@@ -150,6 +151,26 @@ SIGNATURES = {
     "oskar_factorised_batch_config": (None, [P, IP]),
     "oskar_factorised_batch_last_run_ms": (D, [P]),
     "oskar_factorised_batch_free": (None, [P, IP]),
+    # oskar_beam_surrogate.h
+    "oskar_beam_surrogate_num_params": (I, []),
+    "oskar_beam_surrogate_num_features": (I, []),
+    "oskar_beam_surrogate_num_fourier": (I, []),
+    "oskar_beam_surrogate_create": (P, [D, D, D, D, D, C.c_uint, IP]),
+    "oskar_beam_surrogate_free": (None, [P, IP]),
+    "oskar_beam_surrogate_fourier": (None, [P, FP]),
+    "oskar_beam_surrogate_get_params": (None, [P, FP, IP]),
+    "oskar_beam_surrogate_set_params": (None, [P, FP, IP]),
+    "oskar_beam_surrogate_peak": (D, [P]),
+    "oskar_beam_surrogate_set_peak": (None, [P, D]),
+    "oskar_beam_surrogate_training_set": (None, [P, P, I, I, D, D, D, D,
+                                                 C.c_uint, P, P, P, P, IP]),
+    "oskar_beam_surrogate_train": (None, [P, I, P, P, P, P, I, I, D, D,
+                                          C.c_uint, DP, IP]),
+    "oskar_beam_surrogate_loss_and_gradient": (None, [P, I, P, P, DP, P, IP]),
+    "oskar_beam_surrogate_evaluate": (None, [P, I, P, P, D, P, IP]),
+    "oskar_beam_surrogate_evaluate_with_fallback": (None, [P, P, P, I, P, P,
+                                                           P, D, D, P, IP,
+                                                           IP]),
     # oskar_dftw.h
     "oskar_dftw": (None, [I, I, D, P, P, P, P, I, I, P, P, P, P, P, I, I, I,
                           P, IP]),

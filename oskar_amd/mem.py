@@ -309,3 +309,104 @@ class FactorisedBatch:
             self.free()
         except Exception:
             pass
+
+
+class BeamSurrogate:
+    """oskar_BeamSurrogate: the learned station-beam network."""
+
+    def __init__(self, freq_min, freq_max, l0, m0, station_radius, seed=1):
+        st = C.c_int(0)
+        self.ptr = lib.oskar_beam_surrogate_create(
+            float(freq_min), float(freq_max), float(l0), float(m0),
+            float(station_radius), int(seed), C.byref(st))
+        check_status(st, "oskar_beam_surrogate_create")
+        self.num_params = lib.oskar_beam_surrogate_num_params()
+
+    def fourier(self):
+        n = lib.oskar_beam_surrogate_num_fourier()
+        out = np.zeros(2 * n, np.float32)
+        lib.oskar_beam_surrogate_fourier(
+            self.ptr, out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out.reshape(n, 2)
+
+    def get_params(self):
+        out = np.zeros(self.num_params, np.float32)
+        st = C.c_int(0)
+        lib.oskar_beam_surrogate_get_params(
+            self.ptr, out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(st))
+        check_status(st, "oskar_beam_surrogate_get_params")
+        return out
+
+    def set_params(self, theta):
+        a = np.ascontiguousarray(theta, np.float32)
+        st = C.c_int(0)
+        lib.oskar_beam_surrogate_set_params(
+            self.ptr, a.ctypes.data_as(C.POINTER(C.c_float)), C.byref(st))
+        check_status(st, "oskar_beam_surrogate_set_params")
+
+    @property
+    def peak(self):
+        return lib.oskar_beam_surrogate_peak(self.ptr)
+
+    def train(self, n, l, m, f, beam, batch, steps, lr0, lr1, seed=1):
+        hist = np.zeros(max(steps, 1))
+        st = C.c_int(0)
+        lib.oskar_beam_surrogate_train(
+            self.ptr, int(n), l.ptr, m.ptr, f.ptr, beam.ptr, int(batch),
+            int(steps), float(lr0), float(lr1), int(seed),
+            hist.ctypes.data_as(C.POINTER(C.c_double)), C.byref(st))
+        check_status(st, "oskar_beam_surrogate_train")
+        return hist[:steps]
+
+    def loss_and_gradient(self, features, targets):
+        """features [B, 128], targets [B, 8] (numpy); returns (loss, grad)."""
+        B = features.shape[0]
+        fm = Mem.from_numpy(features.ravel(), OSKAR_SINGLE)
+        tm = Mem.from_numpy(targets.ravel(), OSKAR_SINGLE)
+        gm = Mem(OSKAR_SINGLE, OSKAR_GPU, self.num_params)
+        loss = C.c_double(0)
+        st = C.c_int(0)
+        lib.oskar_beam_surrogate_loss_and_gradient(
+            self.ptr, int(B), fm.ptr, tm.ptr, C.byref(loss), gm.ptr,
+            C.byref(st))
+        check_status(st, "oskar_beam_surrogate_loss_and_gradient")
+        return loss.value, gm.numpy()
+
+    def evaluate(self, n, l, m, freq, beam):
+        st = C.c_int(0)
+        lib.oskar_beam_surrogate_evaluate(self.ptr, int(n), l.ptr, m.ptr,
+                                          float(freq), beam.ptr, C.byref(st))
+        check_status(st, "oskar_beam_surrogate_evaluate")
+
+    def evaluate_with_fallback(self, station, work, n, x, y, z, freq,
+                               threshold, beam):
+        st = C.c_int(0)
+        num = C.c_int(0)
+        lib.oskar_beam_surrogate_evaluate_with_fallback(
+            self.ptr, station.ptr, work.ptr, int(n), x.ptr, y.ptr, z.ptr,
+            float(freq), float(threshold), beam.ptr, C.byref(num),
+            C.byref(st))
+        check_status(st, "oskar_beam_surrogate_evaluate_with_fallback")
+        return num.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.oskar_beam_surrogate_free(self.ptr, C.byref(C.c_int(0)))
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def beam_training_set(station, work, num_freqs, dirs_per_freq, freq_min,
+                      freq_max, l0, m0, seed=1):
+    """Exact-beam samples (GPU Mems l, m, f, beam) for surrogate training."""
+    l, m, f = (Mem(OSKAR_SINGLE, OSKAR_GPU, 1) for _ in range(3))
+    beam = Mem(OSKAR_SINGLE | OSKAR_COMPLEX | OSKAR_MATRIX, OSKAR_GPU, 1)
+    st = C.c_int(0)
+    lib.oskar_beam_surrogate_training_set(
+        station.ptr, work.ptr, int(num_freqs), int(dirs_per_freq),
+        float(freq_min), float(freq_max), float(l0), float(m0), int(seed),
+        l.ptr, m.ptr, f.ptr, beam.ptr, C.byref(st))
+    check_status(st, "oskar_beam_surrogate_training_set")
+    return l, m, f, beam
