@@ -43,6 +43,37 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(num_stations, num_sources):
+    """HBM bytes per launch of the MFMA product kernel at this workload,
+    from the committed PMC summary (FETCH_SIZE + WRITE_SIZE of the kernel,
+    each scaled by the same counter's reading for a copy of known size);
+    None if there is no summary for this size."""
+    path = os.path.join(ROOT, "profiles",
+                        "fact_xcorr_kernel_512st_1Msrc_traffic.md")
+    if num_stations != 512 or num_sources != 1_000_000 or \
+            not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def copy_calibration():
+    """Five elementwise copies of 1 GiB (y = x + 0, one kernel reading 1 GiB
+    and writing 1 GiB each), for scaling the FETCH_SIZE / WRITE_SIZE
+    counters to bytes."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n,
+                      "copies": 5}), flush=True)
+
+
 def power_law(rng, n, index=-1.8, lo=0.01, hi=10.0):
     """Inverse CDF of p(S) ~ S^index on [lo, hi]
     (oskar/math/src/oskar_random_power_law.c:15-36)."""
@@ -90,7 +121,12 @@ def main():
     ap.add_argument("--no-batch-tune", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=20000,
                     help="training steps of the beam surrogate; 0 = off")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the copy used to scale PMC byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
               "(no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -182,20 +218,25 @@ def main():
     t_beam = timed(step_beam, args.steps, 1, "beam")
     log("stage ms: prepare %.3f product %.3f reduce %.3f beam %.3f" % (
         stage[0], stage[1], stage[2], 1e3 * t_beam / args.steps))
-    num_tiles = sum(min((N + 63) // 64,
-                        (min(128 * i + 127, N - 1) - 1) // 64 + 1)
-                    for i in range((N + 127) // 128))
+    shape = lib.oskar_cross_correlate_factorised_set_tile_shape(-1)
+    tp, tq = {0: (128, 64), 1: (64, 64), 2: (128, 32), 3: (64, 64)}[shape]
+    num_tiles = sum(min((N + tq - 1) // tq,
+                        (min(tp * i + tp - 1, N - 1) - 1) // tq + 1)
+                    for i in range((N + tp - 1) // tp))
     part = 8192
     s_pad = ((S + part - 1) // part) * part
-    exec_flops = num_tiles * float(s_pad) * 128 * 256 * 2 * 2 * 2 * 3
+    # Executed MFMA work: TP x 4 TQ outputs, Re and Im, 2 reals per source,
+    # 2 flops per multiply-add, 3 passes (hi*hi, hi*lo, lo*hi).
+    exec_flops = num_tiles * float(s_pad) * tp * 4 * tq * 2 * 2 * 2 * 3
     t_prod = stage[1] * 1e-3
     achieved = 64.0 * evals / t_prod / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2),
                 "peak": PEAK_MFMA_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_MFMA_TFLOPS, 4),
-                "traffic": None,
-                "kernel": "fact_xcorr_kernel<_Float16>" if mode == 0 else
-                "fact_xcorr_kernel<__bf16>",
+                "traffic": measured_traffic(N, S),
+                "kernel": "fact_xcorr_kernel (tile %d x %d, %s)" % (
+                    tp, tq, "f16" if mode == 0 else "bf16"),
+                "tile_shape": shape,
                 "kernel_ms": round(stage[1], 4),
                 "algorithmic_flops_per_eval": 64,
                 "mfma_executed_tflops": round(exec_flops / t_prod / 1e12, 2),

@@ -63,6 +63,13 @@ void oskar_cross_correlate_factorised(
  * (default 8192; multiple of 8).  Returns the previous value. */
 OSKAR_EXPORT int oskar_cross_correlate_factorised_set_part_size(int sources);
 
+/* Tile shape of oskar_cross_correlate_factorised(), as for batches
+ * (0 = 128 x 64, 1 = 64 x 64 (default), 2 = 128 x 32, 3 = 64 x 64 with
+ * separate MFMA and builder waves).  Returns the previous shape; a negative
+ * shape only returns the current one; a shape past the last returns -1 and
+ * changes nothing. */
+OSKAR_EXPORT int oskar_cross_correlate_factorised_set_tile_shape(int shape);
+
 /* Stage timing with HIP events on the launch stream (off by default).
  * last_times() waits for the last profiled call on the current device and
  * returns 0 with ms[0..2] = preparation, MFMA product and reduction times;
@@ -122,7 +129,8 @@ OSKAR_EXPORT
 void oskar_factorised_batch_autotune(oskar_FactorisedBatch* batch,
         int num_units, int* status);
 
-/* Tile shapes: 0 = 128 x 64 stations, 1 = 64 x 64, 2 = 128 x 32.
+/* Tile shapes: 0 = 128 x 64 stations, 1 = 64 x 64, 2 = 128 x 32,
+ * 3 = 64 x 64 with separate MFMA and builder waves (512 threads).
  * src_per_part: sources summed in FP32 per work item (multiple of 8).
  * num_workgroups: persistent grid (0 = compute units x occupancy).
  * use_graph: capture and replay a hipGraph (default 1). */

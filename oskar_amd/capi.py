@@ -140,6 +140,7 @@ SIGNATURES = {
     "oskar_cross_correlate_factorised": (None, [I, I, P, MEMA, MEM3, P, MEM3,
                                                 D, I, I, P, IP]),
     "oskar_cross_correlate_factorised_set_part_size": (I, [I]),
+    "oskar_cross_correlate_factorised_set_tile_shape": (I, [I]),
     "oskar_cross_correlate_factorised_set_profiling": (None, [I]),
     "oskar_cross_correlate_factorised_last_times": (I, [DP]),
     "oskar_factorised_batch_create": (P, [I, I, I, I, IP]),

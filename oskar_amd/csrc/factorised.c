@@ -20,6 +20,7 @@ int oskar_amd_fact_launch(int precision, int mode, int use_casa,
         void* vis, int src_per_part, void* stream);
 
 void oskar_amd_fact_set_profiling(int on);
+int oskar_amd_fact_set_tile_shape(int shape);
 int oskar_amd_fact_last_times(double* ms);
 
 void* oskar_amd_fact_batch_create(int precision, int mode, int num_stations,
@@ -50,6 +51,11 @@ void oskar_cross_correlate_factorised_set_profiling(int on)
 int oskar_cross_correlate_factorised_last_times(double ms[3])
 {
     return oskar_amd_fact_last_times(ms);
+}
+
+int oskar_cross_correlate_factorised_set_tile_shape(int shape)
+{
+    return oskar_amd_fact_set_tile_shape(shape);
 }
 
 int oskar_cross_correlate_factorised_set_part_size(int sources)

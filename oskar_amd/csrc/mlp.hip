@@ -134,19 +134,22 @@ __device__ __forceinline__ void sincos_rev(float r, float* s, float* c)
     *c = __builtin_amdgcn_cosf(rr);
 }
 
-/* Feature k of direction (l, m) at frequency f: k < NF: sin of frequency
- * k; NF <= k < 2 NF: cos of frequency k - NF; then the 16 smooth terms. */
+/* Features K0 .. K0 + COUNT - 1 of direction (l, m) at frequency f:
+ * k < NF: sin of frequency k; NF <= k < 2 NF: cos of frequency k - NF; then
+ * the 16 smooth terms.  The feature range is a compile-time constant, so
+ * every index into the encoding table is too (uniform scalar loads). */
+template <int K0, int COUNT>
 __device__ __forceinline__ void encode_features(const Enc& e, float l,
-        float m, float f, int k0, int count, float* out)
+        float m, float f, float* out)
 {
     const float n = sqrtf(fmaxf(0.0f, 1.0f - l * l - m * m));
     const float s = f / e.f_max;
     const float u = s * (l - e.l0), v = s * (m - e.m0);
     const float fh = (2.0f * f - e.f_min - e.f_max) / (e.f_max - e.f_min);
 #pragma unroll
-    for (int i = 0; i < count; ++i)
+    for (int i = 0; i < COUNT; ++i)
     {
-        const int k = k0 + i;
+        const int k = K0 + i;
         float val;
         if (k < 2 * NF)
         {
@@ -183,6 +186,35 @@ __device__ __forceinline__ void encode_features(const Enc& e, float l,
     }
 }
 
+/* Writes the 8 features K0 .. K0 + 7 of one sample, as BF16, to X[k] and
+ * XT[k * B]. */
+template <int K0>
+__device__ __forceinline__ void encode_store8(const Enc& e, float l, float m,
+        float f, __bf16* X, __bf16* XT, int B)
+{
+    float feat[8];
+    encode_features<K0, 8>(e, l, m, f, feat);
+    __bf16 h[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) h[j] = (__bf16) feat[j];
+    uint4 v;
+    __builtin_memcpy(&v, h, 16);
+    *reinterpret_cast<uint4*>(X + K0) = v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) XT[(size_t) (K0 + j) * B] = h[j];
+}
+
+template <int K0>
+__device__ __forceinline__ void encode_all(const Enc& e, float l, float m,
+        float f, __bf16* X, __bf16* XT, int B)
+{
+    if constexpr (K0 < D_IN)
+    {
+        encode_store8<K0>(e, l, m, f, X, XT, B);
+        encode_all<K0 + 8>(e, l, m, f, X, XT, B);
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Training kernels.                                                       */
 
@@ -194,20 +226,7 @@ void surr_encode_kernel(int B, const float* l, const float* m, const float* f,
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     const int s = idx ? idx[b] : b;
-    const float ls = l[s], ms = m[s], fs = f[s];
-    for (int k = 0; k < D_IN; k += 8)
-    {
-        float feat[8];
-        encode_features(e, ls, ms, fs, k, 8, feat);
-        __bf16 h[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) h[j] = (__bf16) feat[j];
-        uint4 v;
-        __builtin_memcpy(&v, h, 16);
-        *reinterpret_cast<uint4*>(X + (size_t) b * D_IN + k) = v;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) XT[(size_t) (k + j) * B + b] = h[j];
-    }
+    encode_all<0>(e, l[s], m[s], f[s], X + (size_t) b * D_IN, XT + b, B);
 }
 
 /* Given FP32 features [B][128] (the gradient check's input path). */
@@ -497,53 +516,94 @@ __device__ __forceinline__ float gld_bias(const float* p, int i)
 }
 constexpr int ACT_PLANES = WIDTH / 8;    /* uint4 planes of one activation */
 
-/* One layer, transposed: D[n][m] = sum_k W[n][k] a[m][k] for this wave's 64
- * rows n (2 blocks) and all 128 directions m (4 blocks); K features in
- * LDS planes act_in[k / 8][m]. */
+/* One hidden layer's operands for this wave: the BF16 weight fragments of
+ * its 64 rows n (two 32-row blocks) for every k-step, and the biases of the
+ * rows its lanes hold in the output layout (n_base + 32 i + 8 grp + 4 lh + r).
+ * Loaded into registers ahead of use, so the MFMAs never wait on L2. */
+struct LayerRegs
+{
+    uint4 w[WIDTH / 16][2];
+    float b[2][4][4];
+};
+
 template <int K>
-__device__ __forceinline__ void infer_layer(const __bf16* W, int n_base,
-        const uint4* act_in, f16v acc[2][4])
+__device__ __forceinline__ void load_weights(const __bf16* W, int n_base,
+        LayerRegs& r)
 {
     const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5;
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 4; ++j)
-            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
     const __bf16* w0 = W + (size_t) (n_base + lr) * K + 8 * lh;
     const __bf16* w1 = w0 + (size_t) 32 * K;
 #pragma unroll
-    for (int k = 0; k < K; k += 16)
+    for (int kk = 0; kk < K / 16; ++kk)
     {
-        const uint4 a0 = gld16(w0 + k);
-        const uint4 a1 = gld16(w1 + k);
-        const uint4* pl = act_in + (k / 8 + lh) * INF_ROWS + lr;
+        r.w[kk][0] = gld16(w0 + 16 * kk);
+        r.w[kk][1] = gld16(w1 + 16 * kk);
+    }
+}
+
+__device__ __forceinline__ void load_biases(const float* bias, int n_base,
+        LayerRegs& r)
+{
+    const int lh = (threadIdx.x & 63) >> 5;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int grp = 0; grp < 4; ++grp)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                r.b[i][grp][q] = gld_bias(bias,
+                        n_base + 32 * i + 8 * grp + 4 * lh + q);
+}
+
+/* One layer, transposed: D[n][m] = sum_k W[n][k] a[m][k] for this wave's 64
+ * rows n (2 blocks) and all 128 directions m (4 blocks); K features in LDS
+ * planes act_in[k / 8][m]. */
+template <int K>
+__device__ __forceinline__ void infer_layer(const LayerRegs& r,
+        const uint4* act_in, f16v acc[2][4])
+{
+    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < K / 16; ++kk)
+    {
+        const uint4* pl = act_in + (2 * kk + lh) * INF_ROWS + lr;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
             const uint4 b = pl[j * 32];
-            acc[0][j] = mfma_bf16(a0, b, acc[0][j]);
-            acc[1][j] = mfma_bf16(a1, b, acc[1][j]);
+            acc[0][j] = mfma_bf16(r.w[kk][0], b, acc[0][j]);
+            acc[1][j] = mfma_bf16(r.w[kk][1], b, acc[1][j]);
         }
     }
 }
 
 /* bias + SiLU, BF16, into the next layer's planes: lane (m = 32 j + lr)
- * holds rows n = n_base + 32 i + 8 grp + 4 lh + r, i.e. 4 consecutive
+ * holds rows n = n_base + 32 i + 8 grp + 4 lh + q, i.e. 4 consecutive
  * features: one 8-byte write per (i, j, grp). */
-__device__ __forceinline__ void infer_store(const float* bias, int n_base,
+__device__ __forceinline__ void infer_store(const LayerRegs& r, int n_base,
         f16v acc[2][4], uint4* act_out)
 {
     const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5;
+#pragma unroll
     for (int i = 0; i < 2; ++i)
+#pragma unroll
         for (int grp = 0; grp < 4; ++grp)
         {
             const int n = n_base + 32 * i + 8 * grp + 4 * lh;
-            float bb[4];
-            for (int r = 0; r < 4; ++r) bb[r] = gld_bias(bias, n + r);
+#pragma unroll
             for (int j = 0; j < 4; ++j)
             {
                 __bf16 h[4];
-                for (int r = 0; r < 4; ++r)
-                    h[r] = (__bf16) silu(acc[i][j][4 * grp + r] + bb[r]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    h[q] = (__bf16) silu(acc[i][j][4 * grp + q] +
+                            r.b[i][grp][q]);
                 uint2 pk;
                 __builtin_memcpy(&pk, h, 8);
                 reinterpret_cast<uint2*>(
@@ -563,14 +623,23 @@ void surr_infer_kernel(int num_points, const FP* l, const FP* m,
     const int tid = threadIdx.x, wave = tid >> 6;
     const int lane = tid & 63, lr = lane & 31, lh = lane >> 5;
     const int d0 = blockIdx.x * INF_ROWS;
+    const int n_base = wave * 64;
+    LayerRegs r;
+    /* Layer 0's operands load while the encoding is computed. */
+    load_weights<D_IN>(wb + t.weight_off[0], n_base, r);
+    load_biases(params + t.param_off[0] + WIDTH * D_IN, n_base, r);
 
-    /* Encoding: two threads per direction, 64 features each. */
+    /* Encoding: two threads per direction, 64 features each (the half is
+     * uniform over a wave). */
     {
-        const int r = tid % INF_ROWS, half = tid / INF_ROWS;
-        const int d = min(d0 + r, num_points - 1);
+        const int row = tid % INF_ROWS;
+        const int half = __builtin_amdgcn_readfirstlane(tid / INF_ROWS);
+        const int d = min(d0 + row, num_points - 1);
         float feat[64];
-        encode_features(e, (float) l[d], (float) m[d], freq, half * 64, 64,
-                feat);
+        if (half == 0)
+            encode_features<0, 64>(e, (float) l[d], (float) m[d], freq, feat);
+        else
+            encode_features<64, 64>(e, (float) l[d], (float) m[d], freq, feat);
 #pragma unroll
         for (int p = 0; p < 8; ++p)
         {
@@ -579,22 +648,26 @@ void surr_infer_kernel(int num_points, const FP* l, const FP* m,
             for (int q = 0; q < 8; ++q) h[q] = (__bf16) feat[8 * p + q];
             uint4 v;
             __builtin_memcpy(&v, h, 16);
-            act[0][(half * 8 + p) * INF_ROWS + r] = v;
+            act[0][(half * 8 + p) * INF_ROWS + row] = v;
         }
     }
     __syncthreads();
     f16v acc[2][4];
-    const int n_base = wave * 64;
-    infer_layer<D_IN>(wb + t.weight_off[0], n_base, act[0], acc);
-    infer_store(params + t.param_off[0] + WIDTH * D_IN, n_base, acc, act[1]);
-    __syncthreads();
-    for (int layer = 1; layer < DEPTH; ++layer)
+#pragma unroll
+    for (int layer = 0; layer < DEPTH; ++layer)
     {
         const uint4* in = act[layer & 1];
         uint4* out = act[(layer + 1) & 1];
-        infer_layer<WIDTH>(wb + t.weight_off[layer], n_base, in, acc);
-        infer_store(params + t.param_off[layer] + WIDTH * WIDTH, n_base,
-                acc, out);
+        if (layer == 0) infer_layer<D_IN>(r, in, acc);
+        else infer_layer<WIDTH>(r, in, acc);
+        /* The next layer's weights load during this layer's epilogue and
+         * barrier (the MFMAs above have consumed the registers). */
+        if (layer + 1 < DEPTH)
+            load_weights<WIDTH>(wb + t.weight_off[layer + 1], n_base, r);
+        infer_store(r, n_base, acc, out);
+        if (layer + 1 < DEPTH)
+            load_biases(params + t.param_off[layer + 1] + WIDTH * WIDTH,
+                    n_base, r);
         __syncthreads();
     }
     /* Output layer: 32 rows (8 used) for this wave's 32 directions. */
@@ -602,20 +675,23 @@ void surr_infer_kernel(int num_points, const FP* l, const FP* m,
         const uint4* in = act[DEPTH & 1];
         const __bf16* w = wb + t.weight_off[DEPTH] + (size_t) lr * WIDTH +
                 8 * lh;
+#pragma unroll
+        for (int kk = 0; kk < WIDTH / 16; ++kk) r.w[kk][0] = gld16(w + 16 * kk);
         f16v o;
+#pragma unroll
         for (int v = 0; v < 16; ++v) o[v] = 0.0f;
 #pragma unroll
-        for (int k = 0; k < WIDTH; k += 16)
+        for (int kk = 0; kk < WIDTH / 16; ++kk)
         {
-            const uint4 a = gld16(w + k);
-            const uint4 b = in[(k / 8 + lh) * INF_ROWS + wave * 32 + lr];
-            o = mfma_bf16(a, b, o);
+            const uint4 b = in[(2 * kk + lh) * INF_ROWS + wave * 32 + lr];
+            o = mfma_bf16(r.w[kk][0], b, o);
         }
-        /* Rows n = 4 lh + r (r < 4) are the 8 outputs. */
+        /* Rows n = 4 lh + q (q < 4) are the 8 outputs. */
         const float* bias = params + t.param_off[DEPTH] + N_OUT_PAD * WIDTH;
-        for (int r = 0; r < 4; ++r)
-            out_e[wave * 32 + lr][4 * lh + r] = out_scale * (o[r] +
-                    gld_bias(bias, 4 * lh + r));
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            out_e[wave * 32 + lr][4 * lh + q] = out_scale * (o[q] +
+                    gld_bias(bias, 4 * lh + q));
     }
     __syncthreads();
     if (tid < INF_ROWS && d0 + tid < num_points)

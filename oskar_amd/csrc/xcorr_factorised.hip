@@ -95,9 +95,11 @@ struct Tile
 using Tile0 = Tile<128, 64, 1>;
 using Tile1 = Tile<64, 64, 2>;
 using Tile2 = Tile<128, 32, 1>;
-constexpr int NUM_SHAPES = 3;
-constexpr int SHAPE_TP[NUM_SHAPES] = {128, 64, 128};
-constexpr int SHAPE_TQ[NUM_SHAPES] = {64, 64, 32};
+/* Shape 3 is the 64 x 64 geometry of Tile1 run by the wave-specialised
+ * body (512 threads). */
+constexpr int NUM_SHAPES = 4;
+constexpr int SHAPE_TP[NUM_SHAPES] = {128, 64, 128, 64};
+constexpr int SHAPE_TQ[NUM_SHAPES] = {64, 64, 32, 64};
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
@@ -307,40 +309,56 @@ __device__ __forceinline__ void load_step(const TileJob& J, int sb,
     }
 }
 
+/* The operand builder of one thread (bt: its index among 256 builder
+ * threads): the station constants of its P and W rows, and the build of one
+ * k-step into a stage buffer.  Stations past N get a zero mask (their rows
+ * build as zeros). */
 template <typename T16, class TL>
-__device__ __forceinline__ void tile_part(const TileJob& J,
-        uint4 (*lds)[TL::STAGE])
+struct Builder
 {
-    constexpr int TP = TL::TP, TQ = TL::TQ, TW = TL::TW;
-    constexpr int PS = TL::PS, QS = TL::QS, RF = TL::RF, CF = TL::CF;
-    constexpr int P_ENTRIES = TL::P_ENTRIES;
-    const int N = J.N;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    static constexpr int TP = TL::TP, TQ = TL::TQ, TW = TL::TW;
+    static constexpr int PS = TL::PS, QS = TL::QS;
+    int pb, pg, qb, qg, p_half, p_off, q_half, q_off, s_last;
+    double pu, pv, pw, qu, qv, qw;
+    float p_mask, q_mask;
 
-    /* Station constants for this thread's P and W builds; stations past N
-     * get a zero mask (their rows build as zeros). */
-    const int pb = tid % TP, pg = tid / TP;       /* P: station, group */
-    const int qb = tid % TQ, qg = tid / TQ;       /* W: station, group */
-    const int ps = min(J.p0 + pb, N - 1), qs = min(J.q0 + qb, N - 1);
-    const double pu = gld(J.st_uvw + 3 * ps), pv = gld(J.st_uvw + 3 * ps + 1),
-            pw = gld(J.st_uvw + 3 * ps + 2);
-    const double qu = gld(J.st_uvw + 3 * qs), qv = gld(J.st_uvw + 3 * qs + 1),
-            qw = gld(J.st_uvw + 3 * qs + 2);
-    const float p_mask = (J.p0 + pb < N) ? K_SCALE : 0.0f;
-    const float q_mask = (J.q0 + qb < N) ? J.scale : 0.0f;
-    /* Position of this thread's sources in the k-step: plane half and
-     * 16-bit element offset within the 16-byte entry. */
-    const int p_half = (pg * PS) / 4, p_off = ((pg * PS) % 4) * 2;
-    const int q_half = (qg * QS) / 4, q_off = ((qg * QS) % 4) * 2;
+    __device__ __forceinline__ Builder(const TileJob& J, int bt)
+    {
+        const int N = J.N;
+        pb = bt % TP;
+        pg = bt / TP;
+        qb = bt % TQ;
+        qg = bt / TQ;
+        const int ps = min(J.p0 + pb, N - 1), qs = min(J.q0 + qb, N - 1);
+        pu = gld(J.st_uvw + 3 * ps);
+        pv = gld(J.st_uvw + 3 * ps + 1);
+        pw = gld(J.st_uvw + 3 * ps + 2);
+        qu = gld(J.st_uvw + 3 * qs);
+        qv = gld(J.st_uvw + 3 * qs + 1);
+        qw = gld(J.st_uvw + 3 * qs + 2);
+        p_mask = (J.p0 + pb < N) ? K_SCALE : 0.0f;
+        q_mask = (J.q0 + qb < N) ? J.scale : 0.0f;
+        /* Position of this thread's sources in the k-step: plane half and
+         * 16-bit element offset within the 16-byte entry. */
+        p_half = (pg * PS) / 4;
+        p_off = ((pg * PS) % 4) * 2;
+        q_half = (qg * QS) / 4;
+        q_off = ((qg * QS) % 4) * 2;
+        s_last = J.s_end - 1;
+    }
 
-    const int s_begin = J.s_begin, s_last = J.s_end - 1;
-    const int nsteps = (J.s_end - s_begin + KSRC - 1) / KSRC;
+    __device__ __forceinline__ void load(const TileJob& J, int sb,
+            StepSrc<PS, QS>* o) const
+    {
+        load_step(J, sb, s_last, pg, qg, o);
+    }
 
     /* Builds the P and W operand planes of the k-step starting at source sb
      * from the prefetched source data, into one stage buffer.  Branch-free:
-     * sources past the end of this part are masked to zero. */
-    auto build = [&](int sb, const StepSrc<PS, QS>& d, uint4* st) {
+     * sources past the end of the part are masked to zero. */
+    __device__ __forceinline__ void build(int sb, const StepSrc<PS, QS>& d,
+            uint4* st) const
+    {
         {
             T16 xh[2 * PS], xl[2 * PS], yh[2 * PS], yl[2 * PS];
 #pragma unroll
@@ -396,13 +414,46 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
             for (int ik = 0; ik < 4; ++ik)
             {
                 const int row = ik * TQ + qb;
-                put<2 * QS>(&st[P_ENTRIES + (0 * 2 + q_half) * TW + row],
+                put<2 * QS>(&st[TL::P_ENTRIES + (0 * 2 + q_half) * TW + row],
                         q_off, wh[ik]);
-                put<2 * QS>(&st[P_ENTRIES + (1 * 2 + q_half) * TW + row],
+                put<2 * QS>(&st[TL::P_ENTRIES + (1 * 2 + q_half) * TW + row],
                         q_off, wl[ik]);
             }
         }
-    };
+    }
+};
+
+/* Stores a wave's FP32 accumulators of one part: element (row i, col j) of
+ * a 32x32 block sits in lane (j + 32 * ((i / 4) % 2)), register
+ * 4 * (i / 8) + i % 4.  Columns are W rows, ordered ik * TQ + q. */
+template <int TP, int TW>
+__device__ __forceinline__ void store_block(float* base, int row0, int col0,
+        const f16v& re, const f16v& im)
+{
+    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5;
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+    {
+        const int row = row0 + 8 * (v / 4) + 4 * lh + (v % 4);
+        const int col = col0 + lr;
+        gst(base + (size_t) row * TW + col, (float) re[v]);
+        gst(base + (size_t) (TP * TW) + (size_t) row * TW + col,
+                (float) im[v]);
+    }
+}
+
+template <typename T16, class TL>
+__device__ __forceinline__ void tile_part(const TileJob& J,
+        uint4 (*lds)[TL::STAGE])
+{
+    constexpr int TP = TL::TP, TW = TL::TW;
+    constexpr int PS = TL::PS, QS = TL::QS, RF = TL::RF, CF = TL::CF;
+    constexpr int P_ENTRIES = TL::P_ENTRIES;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const Builder<T16, TL> bld(J, tid);
+    const int s_begin = J.s_begin;
+    const int nsteps = (J.s_end - s_begin + KSRC - 1) / KSRC;
 
     f16v re[RF][CF], im[RF][CF];
 #pragma unroll
@@ -417,9 +468,9 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
     StepSrc<PS, QS> cur;
     if (nsteps > 0)
     {
-        load_step(J, s_begin, s_last, pg, qg, &cur);
-        build(s_begin, cur, lds[0]);
-        load_step(J, s_begin + KSRC, s_last, pg, qg, &cur);
+        bld.load(J, s_begin, &cur);
+        bld.build(s_begin, cur, lds[0]);
+        bld.load(J, s_begin + KSRC, &cur);
     }
     __syncthreads();
     for (int ks = 0; ks < nsteps; ++ks)
@@ -445,8 +496,7 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
             wh[cf] = st[P_ENTRIES + (0 * 2 + lh) * TW + row];
             wl[cf] = st[P_ENTRIES + (1 * 2 + lh) * TW + row];
         }
-        /* hi*hi, then hi*lo, then lo*hi: 2 RF CF independent MFMAs between
-         * successive updates of one accumulator. */
+        /* hi*hi, then hi*lo, then lo*hi. */
 #pragma unroll
         for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
@@ -473,11 +523,11 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
             }
         /* Build step ks+1 (a masked dummy after the last step) into the
          * other stage, then start the loads for step ks+2. */
-        build(s_begin + (ks + 1) * KSRC, cur, lds[(ks + 1) & 1]);
-        load_step(J, s_begin + (ks + 2) * KSRC, s_last, pg, qg, &cur);
+        bld.build(s_begin + (ks + 1) * KSRC, cur, lds[(ks + 1) & 1]);
+        bld.load(J, s_begin + (ks + 2) * KSRC, &cur);
         /* Schedule: operand reads, then each MFMA followed by a few VALU
          * instructions of the build, then the builder's LDS writes and the
-         * next loads.  The matrix pipe runs while the VALU builds. */
+         * next loads. */
         __builtin_amdgcn_sched_group_barrier(0x100, 4 * RF + 2 * CF, 0);
 #pragma unroll
         for (int i = 0; i < TL::NMFMA; ++i)
@@ -489,25 +539,134 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
         __builtin_amdgcn_sched_group_barrier(0x020, 2 * PS + 3 * QS, 0);
         __syncthreads();
     }
-
-    /* Store this part's FP32 tile: element (row i, col j) of a 32x32 block
-     * sits in lane (j + 32 * ((i / 4) % 2)), register 4 * (i / 8) + i % 4.
-     * Columns are W rows, ordered ik * TQ + q. */
-    float* base = J.out;
 #pragma unroll
     for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
         for (int cf = 0; cf < CF; ++cf)
-#pragma unroll
-            for (int v = 0; v < 16; ++v)
+            store_block<TP, TW>(J.out, wp * (TP / 2) + rf * 32,
+                    ww * (TW / 2) + cf * 32, re[rf][cf], im[rf][cf]);
+}
+
+/* Wave-specialised tile body: the 64 x 64 tile of Tile1 with 512 threads.
+ * Waves 0-3 (one per SIMD) only read operands and issue MFMAs; waves 4-7
+ * (one per SIMD) only build operands.  Each SIMD interleaves the two
+ * instruction streams, so the builder's VALU work runs while the matrix
+ * pipe works, and a stall of one stream does not stop the other.  Three LDS
+ * stages: during step k the MFMA waves multiply operands already in
+ * registers and read step k+1's, while the builder waves write step k+2.
+ * Both roles pass the same number of barriers (1 + steps). */
+constexpr int WS_THREADS = 512;
+constexpr int WS_STAGES = 3;
+
+template <typename T16>
+__device__ __forceinline__ void tile_part_ws(const TileJob& J,
+        uint4 (*lds)[Tile1::STAGE])
+{
+    using TL = Tile1;
+    constexpr int TP = TL::TP, TW = TL::TW, P_ENTRIES = TL::P_ENTRIES;
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int s0 = J.s_begin;
+    const int nsteps = (J.s_end - s0 + KSRC - 1) / KSRC;
+    if (wave >= 4)
+    {
+        const Builder<T16, TL> bld(J, tid - 256);
+        StepSrc<TL::PS, TL::QS> cur;
+        if (nsteps > 0)
+        {
+            bld.load(J, s0, &cur);
+            bld.build(s0, cur, lds[0]);
+        }
+        if (nsteps > 1)
+        {
+            bld.load(J, s0 + KSRC, &cur);
+            bld.build(s0 + KSRC, cur, lds[1]);
+        }
+        bld.load(J, s0 + 2 * KSRC, &cur);
+        __syncthreads();
+        for (int ks = 0; ks < nsteps; ++ks)
+        {
+            if (ks + 2 < nsteps)
             {
-                const int i = 8 * (v / 4) + 4 * lh + (v % 4);
-                const int row = wp * (TP / 2) + rf * 32 + i;
-                const int col = ww * (TW / 2) + cf * 32 + lr;
-                gst(base + (size_t) row * TW + col, (float) re[rf][cf][v]);
-                gst(base + (size_t) (TP * TW) + (size_t) row * TW + col,
-                        (float) im[rf][cf][v]);
+                bld.build(s0 + (ks + 2) * KSRC, cur, lds[(ks + 2) % WS_STAGES]);
+                bld.load(J, s0 + (ks + 3) * KSRC, &cur);
             }
+            __syncthreads();
+        }
+        return;
+    }
+    const int lane = tid & 63, lr = lane & 31, lh = lane >> 5;
+    const int prow = (wave & 1) * 32 + lr;
+    const int wcol = (wave >> 1) * 128 + lr;
+    f16v re[4], im[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) { re[b][v] = 0.0f; im[b][v] = 0.0f; }
+    /* Operand sets: [0..3] xh, xl, yh, yl; [4..7] wh; [8..11] wl. */
+    uint4 opa[12], opb[12];
+    auto read = [&](const uint4* st, uint4* o) {
+        o[0] = st[(0 * 2 + lh) * TP + prow];
+        o[1] = st[(1 * 2 + lh) * TP + prow];
+        o[2] = st[(2 * 2 + lh) * TP + prow];
+        o[3] = st[(3 * 2 + lh) * TP + prow];
+#pragma unroll
+        for (int cf = 0; cf < 4; ++cf)
+        {
+            o[4 + cf] = st[P_ENTRIES + (0 * 2 + lh) * TW + wcol + cf * 32];
+            o[8 + cf] = st[P_ENTRIES + (1 * 2 + lh) * TW + wcol + cf * 32];
+        }
+    };
+    /* hi*hi, hi*lo, lo*hi: 8 independent MFMAs between successive updates
+     * of one accumulator. */
+    auto multiply = [&](const uint4* o) {
+#pragma unroll
+        for (int cf = 0; cf < 4; ++cf)
+        {
+            re[cf] = mfma<T16>(o[0], o[4 + cf], re[cf]);
+            im[cf] = mfma<T16>(o[2], o[4 + cf], im[cf]);
+        }
+#pragma unroll
+        for (int cf = 0; cf < 4; ++cf)
+        {
+            re[cf] = mfma<T16>(o[0], o[8 + cf], re[cf]);
+            im[cf] = mfma<T16>(o[2], o[8 + cf], im[cf]);
+        }
+#pragma unroll
+        for (int cf = 0; cf < 4; ++cf)
+        {
+            re[cf] = mfma<T16>(o[1], o[4 + cf], re[cf]);
+            im[cf] = mfma<T16>(o[3], o[4 + cf], im[cf]);
+        }
+    };
+    /* The next step's reads are issued before this step's MFMAs (the
+     * scheduling barriers keep them there), so they complete under the
+     * matrix work rather than at the next barrier. */
+    __syncthreads();
+    if (nsteps > 0) read(lds[0], opa);
+    /* lgkmcnt(0): the first operands are in registers before the loop, so
+     * the loop body waits only for what it reads itself. */
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    for (int ks = 0; ks < nsteps; ks += 2)
+    {
+        if (ks + 1 < nsteps) read(lds[(ks + 1) % WS_STAGES], opb);
+        __builtin_amdgcn_sched_barrier(0);
+        multiply(opa);
+        __builtin_amdgcn_sched_barrier(0);
+        __syncthreads();
+        if (ks + 1 < nsteps)
+        {
+            if (ks + 2 < nsteps) read(lds[(ks + 2) % WS_STAGES], opa);
+            __builtin_amdgcn_sched_barrier(0);
+            multiply(opb);
+            __builtin_amdgcn_sched_barrier(0);
+            __syncthreads();
+        }
+    }
+    const int row0 = (wave & 1) * 32, col0 = (wave >> 1) * 128;
+#pragma unroll
+    for (int cf = 0; cf < 4; ++cf)
+        store_block<TP, TW>(J.out, row0, col0 + cf * 32, re[cf], im[cf]);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -644,11 +803,39 @@ struct FactArgs
     float* partial;              /* [part][tile][re/im][TP][TW] */
 };
 
-template <typename T16, class TL>
-__global__ __launch_bounds__(THREADS, TL::OCC)
+/* Tile bodies: geometry (TL), workgroup size, occupancy and LDS of the
+ * one-role body (tile_part, 256 threads) and the wave-specialised body
+ * (tile_part_ws, 512 threads, 64 x 64 geometry). */
+template <typename T16, class TL_>
+struct BodyOneRole
+{
+    using TL = TL_;
+    static constexpr int THREADS_ = THREADS, OCC = TL::OCC, STAGES = 2;
+    static __device__ __forceinline__ void run(const TileJob& J,
+            uint4 (*lds)[TL::STAGE])
+    {
+        tile_part<T16, TL>(J, lds);
+    }
+};
+
+template <typename T16>
+struct BodyWS
+{
+    using TL = Tile1;
+    static constexpr int THREADS_ = WS_THREADS, OCC = 1, STAGES = WS_STAGES;
+    static __device__ __forceinline__ void run(const TileJob& J,
+            uint4 (*lds)[TL::STAGE])
+    {
+        tile_part_ws<T16>(J, lds);
+    }
+};
+
+template <class Body>
+__global__ __launch_bounds__(Body::THREADS_, Body::OCC)
 void fact_xcorr_kernel(FactArgs A)
 {
-    __shared__ uint4 lds[2][TL::STAGE];
+    using TL = typename Body::TL;
+    __shared__ uint4 lds[Body::STAGES][TL::STAGE];
     const int tile = blockIdx.x, part = blockIdx.y;
     int ib, jb;
     tile_coords(tile, A.num_stations, TL::TP, TL::TQ, &ib, &jb);
@@ -664,7 +851,7 @@ void fact_xcorr_kernel(FactArgs A)
     J.scale = c_scale(A.cmax);
     J.out = A.partial + ((size_t) part * A.num_tiles + tile) *
             (size_t) (2 * TL::TP * TL::TW);
-    tile_part<T16, TL>(J, lds);
+    Body::run(J, lds);
 }
 
 template <typename FP>
@@ -799,11 +986,12 @@ void fact_batch_prepare_kernel(const BatchHeader* H, const UnitDesc* D)
  * the device counter until all are done (every workgroup exits when the
  * counter passes total_items).  Items of one part are consecutive, so
  * concurrently running workgroups share source data in L2. */
-template <typename T16, class TL>
-__global__ __launch_bounds__(THREADS, TL::OCC)
+template <class Body>
+__global__ __launch_bounds__(Body::THREADS_, Body::OCC)
 void fact_batch_kernel(const BatchHeader* H, const UnitDesc* D, int* counter)
 {
-    __shared__ uint4 lds[2][TL::STAGE];
+    using TL = typename Body::TL;
+    __shared__ uint4 lds[Body::STAGES][TL::STAGE];
     __shared__ int s_item;
     const int total = H->total_items, nu = H->num_units;
     const int N = H->num_stations, tiles = H->num_tiles;
@@ -834,7 +1022,7 @@ void fact_batch_kernel(const BatchHeader* H, const UnitDesc* D, int* counter)
         J.scale = c_scale(d.cmax);
         J.out = d.partial + ((size_t) part * tiles + tile) *
                 (size_t) (2 * TL::TP * TL::TW);
-        tile_part<T16, TL>(J, lds);
+        Body::run(J, lds);
     }
 }
 
@@ -901,17 +1089,26 @@ inline void mark(int dev, int i, hipStream_t stream)
     (void) hipEventRecord(g_ev[dev][i], stream);
 }
 
-template <class TL>
-hipError_t launch_product(int mode, dim3 grid, hipStream_t stream,
-        const FactArgs& A)
+/* The body of each tile shape, for each 16-bit type. */
+template <int SHAPE, typename T16> struct ShapeBody;
+template <typename T16> struct ShapeBody<0, T16>
+{ using type = BodyOneRole<T16, Tile0>; };
+template <typename T16> struct ShapeBody<1, T16>
+{ using type = BodyOneRole<T16, Tile1>; };
+template <typename T16> struct ShapeBody<2, T16>
+{ using type = BodyOneRole<T16, Tile2>; };
+template <typename T16> struct ShapeBody<3, T16>
+{ using type = BodyWS<T16>; };
+
+int occupancy(int shape)
 {
-    if (mode == 1)
-        hipLaunchKernelGGL((fact_xcorr_kernel<__bf16, TL>), grid,
-                dim3(THREADS), 0, stream, A);
-    else
-        hipLaunchKernelGGL((fact_xcorr_kernel<_Float16, TL>), grid,
-                dim3(THREADS), 0, stream, A);
-    return hipGetLastError();
+    switch (shape)
+    {
+    case 1: return Tile1::OCC;
+    case 2: return Tile2::OCC;
+    case 3: return BodyWS<_Float16>::OCC;
+    default: return Tile0::OCC;
+    }
 }
 
 } /* namespace */
@@ -922,24 +1119,20 @@ extern "C" int oskar_amd_fact_num_tiles(int N)
     return num_tiles(N, Tile0::TP, Tile0::TQ);
 }
 
-extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
-        int num_sources, int num_stations, const void* beam,
-        const void* I, const void* Q, const void* U, const void* V,
-        const void* l, const void* m, const void* n, const void* st_u,
-        const void* st_v, const void* st_w, double frequency_hz,
-        double uv_min_lambda, double uv_max_lambda, int offset_out,
-        void* vis, int src_per_part, void* stream_v)
+namespace {
+
+template <int SHAPE>
+int fact_launch_shape(int dev, int precision, int mode, int use_casa,
+        int S, int N, const void* beam, const void* I, const void* Q,
+        const void* U, const void* V, const void* l, const void* m,
+        const void* n, const void* st_u, const void* st_v, const void* st_w,
+        double frequency_hz, double uv_min, double uv_max, int offset_out,
+        void* vis, int src_per_part, hipStream_t stream)
 {
-    using TL = Tile0;
-    const hipStream_t stream = (hipStream_t) stream_v;
-    const int N = num_stations, S = num_sources;
-    if (N < 2 || S <= 0) return 0;
-    int dev = 0;
-    hipError_t err = hipGetDevice(&dev);
-    if (err != hipSuccess) return (int) err;
-    if (dev < 0 || dev >= 64) return (int) hipErrorInvalidDevice;
-    if (src_per_part <= 0) src_per_part = 8192;
-    src_per_part = ((src_per_part + KSRC - 1) / KSRC) * KSRC;
+    using BodyH = typename ShapeBody<SHAPE, _Float16>::type;
+    using BodyB = typename ShapeBody<SHAPE, __bf16>::type;
+    using TL = typename BodyH::TL;
+    hipError_t err = hipSuccess;
     const int parts = (S + src_per_part - 1) / src_per_part;
     const int tiles = num_tiles(N, TL::TP, TL::TQ);
 
@@ -1001,10 +1194,15 @@ extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
     A.cmax = cmax;
     A.partial = partial;
     mark(dev, 1, stream);
-    err = launch_product<TL>(mode, dim3(tiles, parts), stream, A);
+    if (mode == 1)
+        hipLaunchKernelGGL((fact_xcorr_kernel<BodyB>), dim3(tiles, parts),
+                dim3(BodyB::THREADS_), 0, stream, A);
+    else
+        hipLaunchKernelGGL((fact_xcorr_kernel<BodyH>), dim3(tiles, parts),
+                dim3(BodyH::THREADS_), 0, stream, A);
+    err = hipGetLastError();
     if (err != hipSuccess) return (int) err;
 
-    const double uv_min = uv_min_lambda, uv_max = uv_max_lambda;
     mark(dev, 2, stream);
     if (precision == OSKAR_DOUBLE)
         hipLaunchKernelGGL((fact_reduce_kernel<double, TL>), dim3(N),
@@ -1016,6 +1214,55 @@ extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
                 cmax, st, uv_min, uv_max, offset_out, (Cplx<float>*) vis);
     mark(dev, 3, stream);
     return (int) hipGetLastError();
+}
+
+/* Tile shape of the one-call path: 64 x 64 with two workgroups per CU
+ * measured fastest at 512 stations x 1M sources (14.3 ms against 15.7 ms
+ * for 128 x 64 and 16.9 ms for the wave-specialised body). */
+int g_shape = 1;
+
+} /* namespace */
+
+/* Tile shape of oskar_amd_fact_launch (0..3); returns the previous one.  A
+ * negative shape only returns the current one; a shape past the last
+ * returns -1 and changes nothing. */
+extern "C" int oskar_amd_fact_set_tile_shape(int shape)
+{
+    if (shape < 0) return g_shape;
+    if (shape >= NUM_SHAPES) return -1;
+    const int old = g_shape;
+    g_shape = shape;
+    return old;
+}
+
+extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
+        int num_sources, int num_stations, const void* beam,
+        const void* I, const void* Q, const void* U, const void* V,
+        const void* l, const void* m, const void* n, const void* st_u,
+        const void* st_v, const void* st_w, double frequency_hz,
+        double uv_min_lambda, double uv_max_lambda, int offset_out,
+        void* vis, int src_per_part, void* stream_v)
+{
+    const hipStream_t stream = (hipStream_t) stream_v;
+    const int N = num_stations, S = num_sources;
+    if (N < 2 || S <= 0) return 0;
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return (int) err;
+    if (dev < 0 || dev >= 64) return (int) hipErrorInvalidDevice;
+    if (src_per_part <= 0) src_per_part = 8192;
+    src_per_part = ((src_per_part + KSRC - 1) / KSRC) * KSRC;
+#define FACT_ARGS dev, precision, mode, use_casa, S, N, beam, I, Q, U, V, \
+        l, m, n, st_u, st_v, st_w, frequency_hz, uv_min_lambda, \
+        uv_max_lambda, offset_out, vis, src_per_part, stream
+    switch (g_shape)
+    {
+    case 1: return fact_launch_shape<1>(FACT_ARGS);
+    case 2: return fact_launch_shape<2>(FACT_ARGS);
+    case 3: return fact_launch_shape<3>(FACT_ARGS);
+    default: return fact_launch_shape<0>(FACT_ARGS);
+    }
+#undef FACT_ARGS
 }
 
 extern "C" void oskar_amd_fact_set_profiling(int on) { g_profile = on != 0; }
@@ -1045,7 +1292,7 @@ namespace {
 
 struct BatchConfig
 {
-    int shape = 0;               /* index into SHAPE_TP / SHAPE_TQ */
+    int shape = 1;               /* index into SHAPE_TP / SHAPE_TQ */
     int src_per_part = 8192;
     int workgroups = 0;          /* persistent grid; 0 = CUs x occupancy */
     bool operator==(const BatchConfig& o) const
@@ -1075,32 +1322,27 @@ struct Batch
     BatchConfig exec_cfg;
 };
 
-int occupancy(int shape)
-{
-    switch (shape)
-    {
-    case 1: return Tile1::OCC;
-    case 2: return Tile2::OCC;
-    default: return Tile0::OCC;
-    }
-}
-
-template <typename FP, class TL>
+template <typename FP, int SHAPE>
 hipError_t batch_launch_t(Batch* b, hipStream_t s)
 {
+    using BodyH = typename ShapeBody<SHAPE, _Float16>::type;
+    using BodyB = typename ShapeBody<SHAPE, __bf16>::type;
+    using TL = typename BodyH::TL;
     const int wgs = b->cfg.workgroups > 0 ? b->cfg.workgroups :
-            b->num_cus * TL::OCC;
+            b->num_cus * BodyH::OCC;
     const int rows_grid = std::min(4096, b->max_units * b->N);
     hipLaunchKernelGGL(fact_batch_stations_kernel<FP>, dim3(1024),
             dim3(256), 0, s, b->d_hdr, b->d_desc, b->d_counter);
     hipLaunchKernelGGL(fact_batch_prepare_kernel<FP>, dim3(4096), dim3(256),
             0, s, b->d_hdr, b->d_desc);
     if (b->mode == 1)
-        hipLaunchKernelGGL((fact_batch_kernel<__bf16, TL>), dim3(wgs),
-                dim3(THREADS), 0, s, b->d_hdr, b->d_desc, b->d_counter);
+        hipLaunchKernelGGL((fact_batch_kernel<BodyB>), dim3(wgs),
+                dim3(BodyB::THREADS_), 0, s, b->d_hdr, b->d_desc,
+                b->d_counter);
     else
-        hipLaunchKernelGGL((fact_batch_kernel<_Float16, TL>), dim3(wgs),
-                dim3(THREADS), 0, s, b->d_hdr, b->d_desc, b->d_counter);
+        hipLaunchKernelGGL((fact_batch_kernel<BodyH>), dim3(wgs),
+                dim3(BodyH::THREADS_), 0, s, b->d_hdr, b->d_desc,
+                b->d_counter);
     hipLaunchKernelGGL((fact_batch_reduce_kernel<FP, TL>), dim3(rows_grid),
             dim3(256), 0, s, b->d_hdr, b->d_desc);
     return hipGetLastError();
@@ -1111,9 +1353,10 @@ hipError_t batch_launch_fp(Batch* b, hipStream_t s)
 {
     switch (b->cfg.shape)
     {
-    case 1: return batch_launch_t<FP, Tile1>(b, s);
-    case 2: return batch_launch_t<FP, Tile2>(b, s);
-    default: return batch_launch_t<FP, Tile0>(b, s);
+    case 1: return batch_launch_t<FP, 1>(b, s);
+    case 2: return batch_launch_t<FP, 2>(b, s);
+    case 3: return batch_launch_t<FP, 3>(b, s);
+    default: return batch_launch_t<FP, 0>(b, s);
     }
 }
 
@@ -1398,7 +1641,10 @@ extern "C" int oskar_amd_fact_batch_last_ms(void* h, double* ms)
 /* Times every candidate configuration (tile shape x part size x grid) on
  * the first num_units units, with outputs sent to a scratch buffer (the
  * callers' visibilities are not touched), and keeps the fastest.  Direct
- * launches, one warm-up and `reps` timed runs each (median). */
+ * launches, one warm-up and `reps` timed runs each (median).  Part sizes
+ * stop at 8192 sources: the FP32 accumulation error grows about as the
+ * square root of the part size (measured 3e-6 at 8192, up to 9e-6 at
+ * 16384), and 8192 keeps a factor of two below 1e-5. */
 extern "C" int oskar_amd_fact_batch_autotune(void* h, int num_units,
         int reps, void* stream_v, double* best_ms)
 {
@@ -1417,10 +1663,10 @@ extern "C" int oskar_amd_fact_batch_autotune(void* h, int num_units,
     b->use_graph = 0;
     BatchConfig best = saved;
     double best_t = 1e300;
-    static const int parts[] = {4096, 8192, 16384};
+    static const int parts[] = {4096, 8192};
     for (int shape = 0; shape < NUM_SHAPES && err == hipSuccess; ++shape)
     {
-        for (int pi = 0; pi < 3 && err == hipSuccess; ++pi)
+        for (int pi = 0; pi < 2 && err == hipSuccess; ++pi)
         {
             for (int occ = 1; occ <= occupancy(shape) && err == hipSuccess;
                     ++occ)

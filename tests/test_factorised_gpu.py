@@ -73,3 +73,25 @@ def test_factorised_part_size_does_not_change_result_beyond_tolerance(hip):
         lib.oskar_cross_correlate_factorised_set_part_size(old)
     b = run(c, "f", 0, freq)
     assert rel_rms(a, want) < 1e-5 and rel_rms(b, want) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1], ids=["fp16x3", "bf16x3"])
+def test_factorised_every_tile_shape(hip, shape, mode):
+    """The one-call path with each tile shape (ragged station count, source
+    count off the k-step and part sizes, FP32 and FP64)."""
+    from oskar_amd import lib
+    print("factorised shape", shape, "mode", mode, flush=True)
+    c = make_case(77 + shape, 150, 9001)
+    freq = 180e6
+    old = lib.oskar_cross_correlate_factorised_set_tile_shape(shape)
+    try:
+        for prec in ("f", "d"):
+            want = oracle_direct(rounded(c, prec), freq)
+            got = run(c, prec, mode, freq)
+            err = rel_rms(got, want)
+            print("  prec %s rel RMS %.3e" % (prec, err), flush=True)
+            assert err < TOL[mode]
+    finally:
+        lib.oskar_cross_correlate_factorised_set_tile_shape(old)
+    assert lib.oskar_cross_correlate_factorised_set_tile_shape(9) == -1
