@@ -219,7 +219,8 @@ def main():
     log("stage ms: prepare %.3f product %.3f reduce %.3f beam %.3f" % (
         stage[0], stage[1], stage[2], 1e3 * t_beam / args.steps))
     shape = lib.oskar_cross_correlate_factorised_set_tile_shape(-1)
-    tp, tq = {0: (128, 64), 1: (64, 64), 2: (128, 32), 3: (64, 64)}[shape]
+    tp, tq = {0: (128, 64), 1: (64, 64), 2: (128, 32), 3: (64, 64),
+              4: (128, 64)}[shape]
     num_tiles = sum(min((N + tq - 1) // tq,
                         (min(tp * i + tp - 1, N - 1) - 1) // tq + 1)
                     for i in range((N + tp - 1) // tp))

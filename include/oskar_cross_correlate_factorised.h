@@ -65,9 +65,9 @@ OSKAR_EXPORT int oskar_cross_correlate_factorised_set_part_size(int sources);
 
 /* Tile shape of oskar_cross_correlate_factorised(), as for batches
  * (0 = 128 x 64, 1 = 64 x 64 (default), 2 = 128 x 32, 3 = 64 x 64 with
- * separate MFMA and builder waves).  Returns the previous shape; a negative
- * shape only returns the current one; a shape past the last returns -1 and
- * changes nothing. */
+ * separate MFMA and builder waves, 4 = 128 x 64 with 512 threads).
+ * Returns the previous shape; a negative shape only returns the current
+ * one; a shape past the last returns -1 and changes nothing. */
 OSKAR_EXPORT int oskar_cross_correlate_factorised_set_tile_shape(int shape);
 
 /* Stage timing with HIP events on the launch stream (off by default).
@@ -130,7 +130,8 @@ void oskar_factorised_batch_autotune(oskar_FactorisedBatch* batch,
         int num_units, int* status);
 
 /* Tile shapes: 0 = 128 x 64 stations, 1 = 64 x 64, 2 = 128 x 32,
- * 3 = 64 x 64 with separate MFMA and builder waves (512 threads).
+ * 3 = 64 x 64 with separate MFMA and builder waves (512 threads),
+ * 4 = 128 x 64 with 512 threads (two waves per SIMD).
  * src_per_part: sources summed in FP32 per work item (multiple of 8).
  * num_workgroups: persistent grid (0 = compute units x occupancy).
  * use_graph: capture and replay a hipGraph (default 1). */

@@ -228,6 +228,26 @@ OSKAR_EXPORT
 void oskar_blank_below_horizon(int offset_mask, int num_sources,
         const oskar_Mem* mask, int offset_out, oskar_Mem* jones, int* status);
 
+/* oskar/telescope/station/oskar_station_beam.h
+ * (src/oskar_station_beam.c:37-251): the beam of one station for sources in
+ * any supported frame (OSKAR_COORDS_RADEC, _AZEL, _ENU_DIR, _REL_DIR about
+ * (ref_lon_rad, ref_lat_rad)) at one time (UTC MJD; GAST from
+ * oskar_convert_mjd_to_gast_fast) and frequency, written to
+ * beam[offset_out ..].  With the station's normalise_final_beam flag (and a
+ * non-isotropic station) every value is divided by the beam in the
+ * normalisation direction (norm_coord_type: RADEC or AZEL).  If the beam
+ * direction is below the horizon the output is zero.  Station types:
+ * aperture array and isotropic; the Gaussian and VLA primary beams and the
+ * ionospheric screen return OSKAR_ERR_FUNCTION_NOT_AVAILABLE. */
+OSKAR_EXPORT
+void oskar_station_beam(oskar_Station* station, oskar_StationWork* work,
+        int source_coord_type, int num_points,
+        const oskar_Mem* const source_coords[3], double ref_lon_rad,
+        double ref_lat_rad, int norm_coord_type, double norm_lon_rad,
+        double norm_lat_rad, int time_index, double time_start_mjd_utc,
+        double time_mjd_utc, double frequency_hz, int offset_out,
+        oskar_Mem* beam, int* status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,6 +152,14 @@ SIGNATURES = {
     "oskar_factorised_batch_config": (None, [P, IP]),
     "oskar_factorised_batch_last_run_ms": (D, [P]),
     "oskar_factorised_batch_free": (None, [P, IP]),
+    # oskar_convert.h
+    "oskar_convert_mjd_to_gmst": (D, [D]),
+    "oskar_equation_of_equinoxes_fast": (D, [D]),
+    "oskar_convert_mjd_to_gast_fast": (D, [D]),
+    "oskar_convert_any_to_enu_directions": (None, [I, I, MEM3, D, D, D, D,
+                                                   MEM3, IP]),
+    "oskar_station_beam": (None, [P, P, I, I, MEM3, D, D, I, D, D, I, D, D,
+                                  D, I, P, IP]),
     # oskar_beam_surrogate.h
     "oskar_beam_surrogate_num_params": (I, []),
     "oskar_beam_surrogate_num_features": (I, []),

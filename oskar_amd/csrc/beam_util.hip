@@ -509,3 +509,81 @@ extern "C" int oskar_amd_jones_K_launch(int precision, int num_sources,
                 (Cplx<float>*) jones);
     return (int) hipGetLastError();
 }
+
+/* ---- Direction conversions to ENU direction cosines -------------------- */
+/* oskar/convert/define_convert_apparent_ra_dec_to_enu_directions.h,
+ * define_convert_az_el_to_enu_directions.h and
+ * define_convert_relative_directions_to_enu_directions.h (the rotation of
+ * relative direction cosines about the reference point into the horizon
+ * frame).  kind: OSKAR_COORDS_RADEC (a = RA, b = Dec; p = lst, sin lat,
+ * cos lat), OSKAR_COORDS_AZEL (a = az, b = el), OSKAR_COORDS_REL_DIR
+ * (a, b, c = l, m, n; p = sin ha0, cos ha0, sin dec0, cos dec0, sin lat,
+ * cos lat). */
+namespace {
+
+struct EnuParams { double p[6]; };
+
+template <typename FP>
+__global__ __launch_bounds__(THREADS)
+void enu_convert_kernel(int kind, int num, const FP* a, const FP* b,
+        const FP* c, EnuParams P, FP* x, FP* y, FP* z)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num) return;
+    if (kind == OSKAR_COORDS_RADEC)
+    {
+        FP sin_ha, cos_ha, sin_dec, cos_dec;
+        sincos_v<FP>((FP) P.p[0] - a[i], &sin_ha, &cos_ha);
+        sincos_v<FP>(b[i], &sin_dec, &cos_dec);
+        const FP sin_lat = (FP) P.p[1], cos_lat = (FP) P.p[2];
+        x[i] = -cos_dec * sin_ha;
+        y[i] = cos_lat * sin_dec - sin_lat * cos_dec * cos_ha;
+        z[i] = sin_lat * sin_dec + cos_lat * cos_dec * cos_ha;
+    }
+    else if (kind == OSKAR_COORDS_AZEL)
+    {
+        FP sin_az, cos_az, sin_el, cos_el;
+        sincos_v<FP>(a[i], &sin_az, &cos_az);
+        sincos_v<FP>(b[i], &sin_el, &cos_el);
+        x[i] = cos_el * sin_az;
+        y[i] = cos_el * cos_az;
+        z[i] = sin_el;
+    }
+    else
+    {
+        const FP l = a[i], m = b[i], n = c[i];
+        const FP sin_ha0 = (FP) P.p[0], cos_ha0 = (FP) P.p[1];
+        const FP sin_dec0 = (FP) P.p[2], cos_dec0 = (FP) P.p[3];
+        const FP sin_lat = (FP) P.p[4], cos_lat = (FP) P.p[5];
+        x[i] = l * cos_ha0 + m * sin_ha0 * sin_dec0 - n * sin_ha0 * cos_dec0;
+        FP t = sin_lat * cos_ha0;
+        y[i] = -l * sin_lat * sin_ha0 + m * (cos_lat * cos_dec0 + t * sin_dec0)
+                + n * (cos_lat * sin_dec0 - t * cos_dec0);
+        t = cos_lat * cos_ha0;
+        z[i] = l * cos_lat * sin_ha0 + m * (sin_lat * cos_dec0 - t * sin_dec0)
+                + n * (sin_lat * sin_dec0 + t * cos_dec0);
+    }
+}
+
+} /* namespace */
+
+extern "C" int oskar_amd_enu_convert_launch(int precision, int kind, int num,
+        const void* a, const void* b, const void* c, const double* params,
+        void* x, void* y, void* z, void* stream)
+{
+    if (num <= 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+    EnuParams P;
+    for (int k = 0; k < 6; ++k) P.p[k] = params[k];
+    if (precision == OSKAR_DOUBLE)
+        hipLaunchKernelGGL(enu_convert_kernel<double>, grid_for(num),
+                dim3(THREADS), 0, s, kind, num, (const double*) a,
+                (const double*) b, (const double*) c, P, (double*) x,
+                (double*) y, (double*) z);
+    else
+        hipLaunchKernelGGL(enu_convert_kernel<float>, grid_for(num),
+                dim3(THREADS), 0, s, kind, num, (const float*) a,
+                (const float*) b, (const float*) c, P, (float*) x,
+                (float*) y, (float*) z);
+    return (int) hipGetLastError();
+}

@@ -105,6 +105,8 @@ struct oskar_StationWork
     oskar_Mem* theta;
     oskar_Mem* phi_x;
     oskar_Mem* phi_y;
+    oskar_Mem* enu[3];        /* source directions (+1 normalisation point) */
+    oskar_Mem* beam_out;      /* beam of those directions */
 };
 
 struct oskar_Timer
@@ -185,6 +187,9 @@ int oskar_amd_jones_K_launch(int precision, int num_sources,
         void* jones, void* stream);
 int oskar_amd_jones_join_launch(int precision, int a_matrix, int b_matrix,
         size_t n, void* out, const void* a, const void* b, void* stream);
+int oskar_amd_enu_convert_launch(int precision, int kind, int num,
+        const void* a, const void* b, const void* c, const double* params,
+        void* x, void* y, void* z, void* stream);
 
 #ifdef __cplusplus
 }

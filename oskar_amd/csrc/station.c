@@ -477,6 +477,9 @@ oskar_StationWork* oskar_station_work_create(int type, int location,
     w->theta = oskar_mem_create(type, location, 0, status);
     w->phi_x = oskar_mem_create(type, location, 0, status);
     w->phi_y = oskar_mem_create(type, location, 0, status);
+    w->enu[0] = oskar_mem_create(type, location, 0, status);
+    w->enu[1] = oskar_mem_create(type, location, 0, status);
+    w->enu[2] = oskar_mem_create(type, location, 0, status);
     return w;
 }
 
@@ -488,6 +491,10 @@ void oskar_station_work_free(oskar_StationWork* w, int* status)
     oskar_mem_free(w->theta, status);
     oskar_mem_free(w->phi_x, status);
     oskar_mem_free(w->phi_y, status);
+    oskar_mem_free(w->enu[0], status);
+    oskar_mem_free(w->enu[1], status);
+    oskar_mem_free(w->enu[2], status);
+    oskar_mem_free(w->beam_out, status);
     free(w);
 }
 

@@ -67,21 +67,24 @@ namespace {
 constexpr int KSRC = 8;          /* sources per k-step */
 constexpr int THREADS = 256;
 
-template <int TP_, int TQ_, int OCC_>
+template <int TP_, int TQ_, int OCC_, int THREADS_ = THREADS>
 struct Tile
 {
     static constexpr int TP = TP_;               /* P stations per tile */
     static constexpr int TQ = TQ_;               /* Q stations per tile */
     static constexpr int TW = 4 * TQ_;           /* W rows per tile */
-    static constexpr int PG = THREADS / TP;      /* P builder groups */
+    static constexpr int NT = THREADS_;          /* threads per workgroup */
+    static constexpr int PG = NT / TP;           /* P builder groups */
     static constexpr int PS = KSRC / PG;         /* sources per P builder */
-    static constexpr int QG = THREADS / TQ;      /* W builder groups */
+    static constexpr int QG = NT / TQ;           /* W builder groups */
     static constexpr int QS = KSRC / QG;         /* sources per W builder */
     static constexpr int P_ENTRIES = 8 * TP;     /* uint4 per stage, P */
     static constexpr int W_ENTRIES = 4 * TW;     /* uint4 per stage, W */
     static constexpr int STAGE = P_ENTRIES + W_ENTRIES;
-    static constexpr int RF = TP / 64;           /* MFMA rows per wave */
-    static constexpr int CF = TW / 64;           /* MFMA columns per wave */
+    static constexpr int WR = NT / 128;          /* waves along P */
+    static constexpr int WC = 2;                 /* waves along W */
+    static constexpr int RF = TP / (32 * WR);    /* MFMA rows per wave */
+    static constexpr int CF = TW / (32 * WC);    /* MFMA columns per wave */
     static constexpr int NMFMA = 6 * RF * CF;    /* per wave per k-step */
     static constexpr int OCC = OCC_;             /* workgroups per CU */
     static_assert(PS >= 1 && PS <= 4 && QS >= 1 && QS <= 4, "tile shape");
@@ -96,10 +99,14 @@ using Tile0 = Tile<128, 64, 1>;
 using Tile1 = Tile<64, 64, 2>;
 using Tile2 = Tile<128, 32, 1>;
 /* Shape 3 is the 64 x 64 geometry of Tile1 run by the wave-specialised
- * body (512 threads). */
-constexpr int NUM_SHAPES = 4;
-constexpr int SHAPE_TP[NUM_SHAPES] = {128, 64, 128, 64};
-constexpr int SHAPE_TQ[NUM_SHAPES] = {64, 64, 32, 64};
+ * body (512 threads).  Shape 4 is the 128 x 64 tile with 512 threads: two
+ * waves per SIMD (each 32 P x 128 W rows, 128 accumulators), so one wave's
+ * build overlaps the other's MFMAs, with the smaller build work per MFMA of
+ * the larger tile. */
+using Tile4 = Tile<128, 64, 1, 512>;
+constexpr int NUM_SHAPES = 5;
+constexpr int SHAPE_TP[NUM_SHAPES] = {128, 64, 128, 64, 128};
+constexpr int SHAPE_TQ[NUM_SHAPES] = {64, 64, 32, 64, 64};
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
@@ -286,6 +293,7 @@ struct StepSrc
 struct TileJob
 {
     int N, p0, q0, s_begin, s_end;
+    int s_clamp;                 /* last valid source slot (padded count - 1) */
     const Src* src;              /* per source: l, m, n - 1 (FP64) */
     const float4* cmat;          /* per source: C00, Re C01, Im C01, C11 */
     const double* st_uvw;        /* per station: U, V, W (revolutions) */
@@ -320,7 +328,7 @@ struct Builder
     static constexpr int PS = TL::PS, QS = TL::QS;
     int pb, pg, qb, qg, p_half, p_off, q_half, q_off, s_last;
     double pu, pv, pw, qu, qv, qw;
-    float p_mask, q_mask;
+    float q_scale;
 
     __device__ __forceinline__ Builder(const TileJob& J, int bt)
     {
@@ -336,15 +344,14 @@ struct Builder
         qu = gld(J.st_uvw + 3 * qs);
         qv = gld(J.st_uvw + 3 * qs + 1);
         qw = gld(J.st_uvw + 3 * qs + 2);
-        p_mask = (J.p0 + pb < N) ? K_SCALE : 0.0f;
-        q_mask = (J.q0 + qb < N) ? J.scale : 0.0f;
+        q_scale = J.scale;
         /* Position of this thread's sources in the k-step: plane half and
          * 16-bit element offset within the 16-byte entry. */
         p_half = (pg * PS) / 4;
         p_off = ((pg * PS) % 4) * 2;
         q_half = (qg * QS) / 4;
         q_off = ((qg * QS) % 4) * 2;
-        s_last = J.s_end - 1;
+        s_last = J.s_clamp;
     }
 
     __device__ __forceinline__ void load(const TileJob& J, int sb,
@@ -354,8 +361,10 @@ struct Builder
     }
 
     /* Builds the P and W operand planes of the k-step starting at source sb
-     * from the prefetched source data, into one stage buffer.  Branch-free:
-     * sources past the end of the part are masked to zero. */
+     * from the prefetched source data, into one stage buffer.  No masks:
+     * the source slots past the last source (up to a multiple of 8) hold
+     * C = 0, so their products are exactly zero, and the rows of stations
+     * past N (built from the last station's constants) are never read. */
     __device__ __forceinline__ void build(int sb, const StepSrc<PS, QS>& d,
             uint4* st) const
     {
@@ -364,12 +373,11 @@ struct Builder
 #pragma unroll
             for (int j = 0; j < PS; ++j)
             {
-                const float mk = (sb + pg * PS + j <= s_last) ? p_mask : 0.0f;
                 float re, im;
                 phasor(pu * d.p[j].l + pv * d.p[j].m + pw * d.p[j].n1, &re,
                         &im);
-                re *= mk;
-                im *= mk;
+                re *= K_SCALE;
+                im *= K_SCALE;
                 split16<T16>(re, &xh[2 * j], &xl[2 * j]);
                 split16<T16>(im, &xh[2 * j + 1], &xl[2 * j + 1]);
                 /* Y = (Im, -Re): negation is exact in the split parts. */
@@ -390,12 +398,12 @@ struct Builder
 #pragma unroll
             for (int j = 0; j < QS; ++j)
             {
-                const float mk = (sb + qg * QS + j <= s_last) ? q_mask : 0.0f;
                 float kr, ki;
                 phasor(qu * d.w[j].l + qv * d.w[j].m + qw * d.w[j].n1, &kr,
                         &ki);
-                const float4 c = make_float4(d.c[j].x * mk, d.c[j].y * mk,
-                        d.c[j].z * mk, d.c[j].w * mk);
+                kr *= q_scale;
+                ki *= q_scale;
+                const float4 c = d.c[j];
                 /* W_ik = K conj(C_ik): C00, C11 real; C10 = conj(C01). */
                 float w[4][2];
                 w[0][0] = kr * c.x;  w[0][1] = ki * c.x;
@@ -428,9 +436,9 @@ struct Builder
  * 4 * (i / 8) + i % 4.  Columns are W rows, ordered ik * TQ + q. */
 template <int TP, int TW>
 __device__ __forceinline__ void store_block(float* base, int row0, int col0,
-        const f16v& re, const f16v& im)
+        const f16v& re, const f16v& im, int tid)
 {
-    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5;
+    const int lane = tid & 63, lr = lane & 31, lh = lane >> 5;
 #pragma unroll
     for (int v = 0; v < 16; ++v)
     {
@@ -442,14 +450,16 @@ __device__ __forceinline__ void store_block(float* base, int row0, int col0,
     }
 }
 
+/* tid: the thread's index in the workgroup (a value the persistent batch
+ * kernel makes opaque per work item, so that nothing derived from it is
+ * hoisted out of its item loop and held in registers across it). */
 template <typename T16, class TL>
 __device__ __forceinline__ void tile_part(const TileJob& J,
-        uint4 (*lds)[TL::STAGE])
+        uint4 (*lds)[TL::STAGE], int tid)
 {
     constexpr int TP = TL::TP, TW = TL::TW;
     constexpr int PS = TL::PS, QS = TL::QS, RF = TL::RF, CF = TL::CF;
     constexpr int P_ENTRIES = TL::P_ENTRIES;
-    const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const Builder<T16, TL> bld(J, tid);
     const int s_begin = J.s_begin;
@@ -463,7 +473,7 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
 #pragma unroll
             for (int v = 0; v < 16; ++v) { re[a][b][v] = 0.0f; im[a][b][v] = 0.0f; }
 
-    const int wp = wave & 1, ww = wave >> 1;
+    const int wp = wave % TL::WR, ww = wave / TL::WR;
     const int lr = lane & 31, lh = lane >> 5;
     StepSrc<PS, QS> cur;
     if (nsteps > 0)
@@ -483,7 +493,7 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
 #pragma unroll
         for (int rf = 0; rf < RF; ++rf)
         {
-            const int row = wp * (TP / 2) + rf * 32 + lr;
+            const int row = wp * (TP / TL::WR) + rf * 32 + lr;
             xh[rf] = st[(0 * 2 + lh) * TP + row];
             xl[rf] = st[(1 * 2 + lh) * TP + row];
             yh[rf] = st[(2 * 2 + lh) * TP + row];
@@ -492,7 +502,7 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
 #pragma unroll
         for (int cf = 0; cf < CF; ++cf)
         {
-            const int row = ww * (TW / 2) + cf * 32 + lr;
+            const int row = ww * (TW / TL::WC) + cf * 32 + lr;
             wh[cf] = st[P_ENTRIES + (0 * 2 + lh) * TW + row];
             wl[cf] = st[P_ENTRIES + (1 * 2 + lh) * TW + row];
         }
@@ -543,8 +553,9 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
     for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
         for (int cf = 0; cf < CF; ++cf)
-            store_block<TP, TW>(J.out, wp * (TP / 2) + rf * 32,
-                    ww * (TW / 2) + cf * 32, re[rf][cf], im[rf][cf]);
+            store_block<TP, TW>(J.out, wp * (TP / TL::WR) + rf * 32,
+                    ww * (TW / TL::WC) + cf * 32, re[rf][cf], im[rf][cf],
+                    tid);
 }
 
 /* Wave-specialised tile body: the 64 x 64 tile of Tile1 with 512 threads.
@@ -560,11 +571,10 @@ constexpr int WS_STAGES = 3;
 
 template <typename T16>
 __device__ __forceinline__ void tile_part_ws(const TileJob& J,
-        uint4 (*lds)[Tile1::STAGE])
+        uint4 (*lds)[Tile1::STAGE], int tid)
 {
     using TL = Tile1;
     constexpr int TP = TL::TP, TW = TL::TW, P_ENTRIES = TL::P_ENTRIES;
-    const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int s0 = J.s_begin;
     const int nsteps = (J.s_end - s0 + KSRC - 1) / KSRC;
@@ -666,7 +676,8 @@ __device__ __forceinline__ void tile_part_ws(const TileJob& J,
     const int row0 = (wave & 1) * 32, col0 = (wave >> 1) * 128;
 #pragma unroll
     for (int cf = 0; cf < 4; ++cf)
-        store_block<TP, TW>(J.out, row0, col0 + cf * 32, re[cf], im[cf]);
+        store_block<TP, TW>(J.out, row0, col0 + cf * 32, re[cf], im[cf],
+                tid);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -719,6 +730,17 @@ __device__ __forceinline__ float prepare_source(int s, const FP* I,
     gst_f4(cmat + s, (float) c00, (float) c01r, (float) c01i, (float) c11);
     return fmaxf(fmaxf(fabsf((float) c00), fabsf((float) c11)),
             fmaxf(fabsf((float) c01r), fabsf((float) c01i)));
+}
+
+/* Source slots are padded to a multiple of the k-step with C = 0. */
+__host__ __device__ inline int pad8(int n) { return (n + KSRC - 1) / KSRC * KSRC; }
+
+__device__ __forceinline__ void zero_source(int s, Src* src, float4* cmat)
+{
+    Src o;
+    o.l = o.m = o.n1 = 0.0;
+    gst_src(src + s, o);
+    gst_f4(cmat + s, 0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 /* Workgroup maximum of non-negative values, then one atomic per workgroup
@@ -810,11 +832,11 @@ template <typename T16, class TL_>
 struct BodyOneRole
 {
     using TL = TL_;
-    static constexpr int THREADS_ = THREADS, OCC = TL::OCC, STAGES = 2;
+    static constexpr int THREADS_ = TL::NT, OCC = TL::OCC, STAGES = 2;
     static __device__ __forceinline__ void run(const TileJob& J,
-            uint4 (*lds)[TL::STAGE])
+            uint4 (*lds)[TL::STAGE], int tid)
     {
-        tile_part<T16, TL>(J, lds);
+        tile_part<T16, TL>(J, lds, tid);
     }
 };
 
@@ -824,9 +846,9 @@ struct BodyWS
     using TL = Tile1;
     static constexpr int THREADS_ = WS_THREADS, OCC = 1, STAGES = WS_STAGES;
     static __device__ __forceinline__ void run(const TileJob& J,
-            uint4 (*lds)[TL::STAGE])
+            uint4 (*lds)[TL::STAGE], int tid)
     {
-        tile_part_ws<T16>(J, lds);
+        tile_part_ws<T16>(J, lds, tid);
     }
 };
 
@@ -845,13 +867,14 @@ void fact_xcorr_kernel(FactArgs A)
     J.q0 = jb * TL::TQ;
     J.s_begin = part * A.src_per_part;
     J.s_end = min(A.num_sources, J.s_begin + A.src_per_part);
+    J.s_clamp = pad8(A.num_sources) - 1;
     J.src = A.src;
     J.cmat = A.cmat;
     J.st_uvw = A.st_uvw;
     J.scale = c_scale(A.cmax);
     J.out = A.partial + ((size_t) part * A.num_tiles + tile) *
             (size_t) (2 * TL::TP * TL::TW);
-    Body::run(J, lds);
+    Body::run(J, lds, threadIdx.x);
 }
 
 template <typename FP>
@@ -865,6 +888,8 @@ void fact_prepare_kernel(int num_sources, const FP* I, const FP* Q,
     float mx = 0.0f;
     if (s < num_sources)
         mx = prepare_source<FP>(s, I, Q, U, V, l, m, n, beam, src, cmat);
+    else if (s < pad8(num_sources))
+        zero_source(s, src, cmat);
     block_max_to(mx, cmax, red);
 }
 
@@ -919,6 +944,14 @@ struct UnitDesc
     int prep_start;              /* first prepare slot (256-aligned) */
     int pad0;
 };
+
+/* x, through an empty asm the compiler must assume changes it: values
+ * derived from the result are computed where they are used. */
+__device__ __forceinline__ int opaque(int x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
 
 /* Last unit u with key(u) <= x (keys non-decreasing; key(0) = 0 <= x).
  * Uniform across the workgroup: the loads are scalar. */
@@ -977,6 +1010,8 @@ void fact_batch_prepare_kernel(const BatchHeader* H, const UnitDesc* D)
                     (const FP*) d.U, (const FP*) d.V, (const FP*) d.l,
                     (const FP*) d.m, (const FP*) d.n,
                     (const Cplx<FP>*) d.beam, d.src, d.cmat);
+        else if (s < pad8(d.num_sources))
+            zero_source(s, d.src, d.cmat);
         block_max_to(mx, d.cmax, red);
         __syncthreads();
     }
@@ -1001,7 +1036,8 @@ void fact_batch_kernel(const BatchHeader* H, const UnitDesc* D, int* counter)
         __syncthreads();          /* every thread has read s_item */
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
-        const int item = s_item;
+        /* Uniform over the workgroup: scalar descriptor loads. */
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
         if (item >= total) break;
         const int u = find_unit(nu, item,
                 [&](int k) { return D[k].item_start; });
@@ -1016,13 +1052,14 @@ void fact_batch_kernel(const BatchHeader* H, const UnitDesc* D, int* counter)
         J.q0 = jb * TL::TQ;
         J.s_begin = part * spp;
         J.s_end = min(d.num_sources, J.s_begin + spp);
+        J.s_clamp = pad8(d.num_sources) - 1;
         J.src = d.src;
         J.cmat = d.cmat;
         J.st_uvw = d.st;
         J.scale = c_scale(d.cmax);
         J.out = d.partial + ((size_t) part * tiles + tile) *
                 (size_t) (2 * TL::TP * TL::TW);
-        Body::run(J, lds);
+        Body::run(J, lds, opaque(threadIdx.x));
     }
 }
 
@@ -1099,6 +1136,8 @@ template <typename T16> struct ShapeBody<2, T16>
 { using type = BodyOneRole<T16, Tile2>; };
 template <typename T16> struct ShapeBody<3, T16>
 { using type = BodyWS<T16>; };
+template <typename T16> struct ShapeBody<4, T16>
+{ using type = BodyOneRole<T16, Tile4>; };
 
 int occupancy(int shape)
 {
@@ -1107,6 +1146,7 @@ int occupancy(int shape)
     case 1: return Tile1::OCC;
     case 2: return Tile2::OCC;
     case 3: return BodyWS<_Float16>::OCC;
+    case 4: return Tile4::OCC;
     default: return Tile0::OCC;
     }
 }
@@ -1140,8 +1180,9 @@ int fact_launch_shape(int dev, int precision, int mode, int use_casa,
      * [cmat S x 16 B]; workspace 1: partials. */
     const size_t off_st = 256;
     const size_t off_src = off_st + align256(3 * (size_t) N * 8);
-    const size_t off_c = off_src + align256(sizeof(Src) * (size_t) S);
-    const size_t bytes0 = off_c + 16 * (size_t) S;
+    const int S8 = pad8(S);
+    const size_t off_c = off_src + align256(sizeof(Src) * (size_t) S8);
+    const size_t bytes0 = off_c + 16 * (size_t) S8;
     char* w0 = (char*) ws_get(&g_ws[dev][0], bytes0, &err);
     if (err != hipSuccess) return (int) err;
     const size_t bytes1 = (size_t) parts * tiles * 2 * TL::TP * TL::TW *
@@ -1162,7 +1203,7 @@ int fact_launch_shape(int dev, int precision, int mode, int use_casa,
                 (const double*) st_u, (const double*) st_v,
                 (const double*) st_w, f_over_c, st, cmax);
         hipLaunchKernelGGL(fact_prepare_kernel<double>,
-                dim3((S + 255) / 256), dim3(256), 0, stream, S,
+                dim3((S8 + 255) / 256), dim3(256), 0, stream, S,
                 (const double*) I, (const double*) Q, (const double*) U,
                 (const double*) V, (const double*) l, (const double*) m,
                 (const double*) n, (const Cplx<double>*) beam, src, cmat,
@@ -1175,7 +1216,7 @@ int fact_launch_shape(int dev, int precision, int mode, int use_casa,
                 (const float*) st_u, (const float*) st_v,
                 (const float*) st_w, f_over_c, st, cmax);
         hipLaunchKernelGGL(fact_prepare_kernel<float>,
-                dim3((S + 255) / 256), dim3(256), 0, stream, S,
+                dim3((S8 + 255) / 256), dim3(256), 0, stream, S,
                 (const float*) I, (const float*) Q, (const float*) U,
                 (const float*) V, (const float*) l, (const float*) m,
                 (const float*) n, (const Cplx<float>*) beam, src, cmat, cmax);
@@ -1260,6 +1301,7 @@ extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
     case 1: return fact_launch_shape<1>(FACT_ARGS);
     case 2: return fact_launch_shape<2>(FACT_ARGS);
     case 3: return fact_launch_shape<3>(FACT_ARGS);
+    case 4: return fact_launch_shape<4>(FACT_ARGS);
     default: return fact_launch_shape<0>(FACT_ARGS);
     }
 #undef FACT_ARGS
@@ -1356,6 +1398,7 @@ hipError_t batch_launch_fp(Batch* b, hipStream_t s)
     case 1: return batch_launch_t<FP, 1>(b, s);
     case 2: return batch_launch_t<FP, 2>(b, s);
     case 3: return batch_launch_t<FP, 3>(b, s);
+    case 4: return batch_launch_t<FP, 4>(b, s);
     default: return batch_launch_t<FP, 0>(b, s);
     }
 }
@@ -1391,7 +1434,8 @@ hipError_t batch_upload(Batch* b, int nu, int use_casa,
         const int parts = (S + spp - 1) / spp;
         off[u] = bytes;
         bytes += 256 + align256(3 * (size_t) N * 8) +
-                align256(sizeof(Src) * (size_t) S) + align256(16 * (size_t) S)
+                align256(sizeof(Src) * (size_t) pad8(S)) +
+                align256(16 * (size_t) pad8(S))
                 + (size_t) parts * tiles * tile_bytes;
         items += (long long) parts * tiles;
         prep += (S + 255) / 256 * 256;
@@ -1435,9 +1479,9 @@ hipError_t batch_upload(Batch* b, int nu, int use_casa,
         d.st = (double*) w;
         w += align256(3 * (size_t) N * 8);
         d.src = (Src*) w;
-        w += align256(sizeof(Src) * (size_t) S);
+        w += align256(sizeof(Src) * (size_t) pad8(S));
         d.cmat = (float4*) w;
-        w += align256(16 * (size_t) S);
+        w += align256(16 * (size_t) pad8(S));
         d.partial = (float*) w;
         d.num_parts = parts;
         d.item_start = item_start;

@@ -148,7 +148,7 @@ def test_batch_equals_one_call_bitwise(hip, prec):
     agree bit for bit too."""
     print("batch vs one call", prec, flush=True)
     first = None
-    for shape in (0, 1, 3):
+    for shape in (0, 1, 3, 4):
         case = BatchCase(prec, 0)
         case.batch.set_config(shape, 8192, 0, 1)
         case.run()
@@ -170,8 +170,8 @@ def test_batch_equals_one_call_bitwise(hip, prec):
 
 @pytest.mark.parametrize("shape,part,mode", [
     (0, 4096, 0), (0, 16384, 0), (1, 8192, 0), (1, 4096, 0), (2, 8192, 0),
-    (2, 16384, 0), (3, 8192, 0), (3, 4096, 0), (0, 8192, 1), (1, 8192, 1),
-    (2, 4096, 1), (3, 16384, 1)])
+    (2, 16384, 0), (3, 8192, 0), (3, 4096, 0), (4, 8192, 0), (4, 4096, 0),
+    (0, 8192, 1), (1, 8192, 1), (2, 4096, 1), (3, 16384, 1), (4, 8192, 1)])
 def test_batch_shapes_and_parts_match_fp64_oracle(hip, shape, part, mode):
     print("batch shape", shape, "part", part, "mode", mode, flush=True)
     case = BatchCase("f", mode)
@@ -248,7 +248,7 @@ def test_batch_autotune_leaves_outputs_and_caches(hip):
     cfg = case.batch.autotune(len(UNITS))
     t_first = time.perf_counter() - t0
     print("  tuned", cfg, "in %.2f s" % t_first, flush=True)
-    assert cfg["tile_shape"] in (0, 1, 2, 3)
+    assert cfg["tile_shape"] in (0, 1, 2, 3, 4)
     assert cfg["src_per_part"] in (4096, 8192)
     got = case.results()
     for u in range(len(UNITS)):
@@ -273,7 +273,7 @@ def test_batch_argument_errors(hip):
     with pytest.raises(OskarError):
         case.batch.run(len(UNITS) + 1, 0, case.tel)
     with pytest.raises(OskarError):
-        case.batch.set_config(4, 8192)
+        case.batch.set_config(5, 8192)
     d = case.dev[0]
     with pytest.raises(OskarError):
         case.batch.set_unit(len(UNITS), 10, None, d["flux"], d["dirs"],

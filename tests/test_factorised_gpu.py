@@ -75,7 +75,7 @@ def test_factorised_part_size_does_not_change_result_beyond_tolerance(hip):
     assert rel_rms(a, want) < 1e-5 and rel_rms(b, want) < 1e-5
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("mode", [0, 1], ids=["fp16x3", "bf16x3"])
 def test_factorised_every_tile_shape(hip, shape, mode):
     """The one-call path with each tile shape (ragged station count, source
