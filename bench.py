@@ -18,6 +18,9 @@ Measured:
   error    relative RMS of both against an FP64 / long-double CPU oracle on
            a random sample of baselines at the full source count;
   roofline the MFMA product kernel, timed with HIP events on its stream;
+  auto     oskar_auto_correlate on the plain pipeline's joined Jones
+           (512 x 1M FP32 2x2 matrices, 16.4 GB), an HBM-bound sum,
+           against the 8 TB/s HBM peak;
   cpu      the reference's own CPU correlator (compiled from its sources into
            oracle/_ref) on a bounded sample of the workload.
 """
@@ -37,6 +40,7 @@ C0 = 299792458.0
 METRIC = ("source·baseline·channel evals/s at 512 stations vs plain "
           "FP32; rel RMS err vs FP64")
 PEAK_MFMA_TFLOPS = 2500.0   # MI355X dense FP16/BF16 MFMA
+PEAK_HBM_GBPS = 8000.0      # MI355X HBM3E
 
 
 def log(*a):
@@ -111,6 +115,9 @@ def main():
     ap.add_argument("--mode", choices=["fp16x3", "bf16x3"], default="fp16x3")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--auto-steps", type=int, default=5,
+                    help="oskar_auto_correlate steps on the plain "
+                         "pipeline's Jones (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--error-baselines", type=int, default=48)
@@ -254,6 +261,7 @@ def main():
     # --- plain FP32 pipeline ------------------------------------------
     plain_rate = None
     vis_plain = None
+    auto = None
     if not args.no_plain:
         E = Jones(CM, OSKAR_GPU, N, S)
         K = Jones(F32 | OSKAR_COMPLEX, OSKAR_GPU, N, S)
@@ -285,6 +293,29 @@ def main():
         step_plain()
         lib.oskar_device_synchronize(OSKAR_GPU)
         vis_plain = vis.numpy()
+
+        # Auto-correlations of the joined Jones: every 2x2 FP32 value is
+        # read once (32 B per station and source), so the sum is HBM-bound.
+        if args.auto_steps > 0:
+            vis_auto = Mem(CM, OSKAR_GPU, N)
+
+            def step_auto():
+                lib.oskar_auto_correlate(
+                    S, E.ptr, capi.MEMA(*[m.ptr for m in flux]), 0,
+                    vis_auto.ptr, C.byref(status))
+            t_auto = timed(step_auto, args.auto_steps, 1, "auto")
+            ms_auto = 1e3 * t_auto / args.auto_steps
+            auto_bytes = 32 * N * S + 4 * 4 * S + 2 * 32 * N
+            gbps = auto_bytes / (ms_auto * 1e-3) / 1e9
+            auto = {"workload": "%d stations x %d sources, FP32 2x2 Jones"
+                                % (N, S),
+                    "ms": round(ms_auto, 4),
+                    "algorithmic_bytes": auto_bytes,
+                    "achieved_GBps": round(gbps, 1),
+                    "hbm_peak_GBps": PEAK_HBM_GBPS,
+                    "frac": round(gbps / PEAK_HBM_GBPS, 4)}
+            log("auto-correlation: %.3f ms, %.0f GB/s" % (ms_auto, gbps))
+            del vis_auto
         del E, K
 
     # --- error against FP64 ------------------------------------------
@@ -417,6 +448,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
         "surrogate": surrogate,
+        "auto_correlate": auto,
     }
     print(json.dumps(line), flush=True)
 

@@ -47,6 +47,9 @@ def lib():
             [I, P, P, P, I, P, P, P, D, P, D, D, P])
         _lib.oracle_factorised_subset_ld.argtypes = (
             [I, P, P, P, P, P, I, P, P, P, D, I, P, P, P])
+        ac = [I, I, I, I] + [P] * 6
+        _lib.oracle_auto_correlate_d.argtypes = ac
+        _lib.oracle_auto_correlate_f.argtypes = ac
     return _lib
 
 
@@ -94,6 +97,27 @@ def cross_correlate(precision, use_casa, source_type, matrix, jones, I_, Q,
        int(bool(azel)), float(ra0), float(dec0), float(uv_min), float(uv_max),
        int(bool(uv_wavelengths)), float(gast), float(freq), int(offset_out),
        _p(vis))
+    return vis
+
+
+def auto_correlate(precision, matrix, num_stations, jones, I_, Q=None,
+                   U=None, V=None, vis=None, offset_out=0):
+    """oskar_auto_correlate semantics on host arrays (jones: stations x
+    sources values, 8 reals per matrix, 2 per scalar); returns vis
+    (accumulated into the given vis if any)."""
+    dt = np.float64 if precision == "double" else np.float32
+    nr = 8 if matrix else 2
+    if vis is None:
+        vis = np.zeros((offset_out + num_stations) * nr, dtype=dt)
+    vis = np.ascontiguousarray(vis, dtype=dt)
+    num_sources = I_.size
+    z = np.zeros(num_sources)
+    arrs = [_c(x, dt) for x in (jones, I_, z if Q is None else Q,
+                                z if U is None else U, z if V is None else V)]
+    fn = (lib().oracle_auto_correlate_d if precision == "double"
+          else lib().oracle_auto_correlate_f)
+    fn(int(matrix), int(num_sources), int(num_stations), int(offset_out),
+       *[_p(x) for x in arrs], _p(vis))
     return vis
 
 

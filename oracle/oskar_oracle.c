@@ -536,6 +536,67 @@ int oracle_station_beam_aa_d(int matrix, int num_elements,
     return 0;
 }
 
+/*
+ * Auto-correlation, oskar/correlate/define_auto_correlate.h:32-58 (matrix)
+ * and :84-104 (scalar), CPU bodies: per station s,
+ *   matrix: sum = sum_i J_s(i) B_i J_s(i)^H (M = J B with B Hermitian, then
+ *           M J^H, as the cross-correlation body), plain sum in FP;
+ *           the imaginary parts of sum.a and sum.d are set to zero, then
+ *           vis[offset_out + s] += sum;
+ *   scalar: sum = sum_i |J_s(i)|^2 I_i; vis[offset_out + s].re += sum.
+ */
+#define ACORR_BODY(NAME, FP)                                                  \
+void NAME(int matrix, int num_sources, int num_stations, int offset_out,     \
+        const FP* jones, const FP* I, const FP* Q, const FP* U, const FP* V,  \
+        FP* vis)                                                              \
+{                                                                             \
+    int s;                                                                    \
+    _Pragma("omp parallel for")                                               \
+    for (s = 0; s < num_stations; ++s)                                        \
+    {                                                                         \
+        FP sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};                                 \
+        int i, r;                                                             \
+        if (!matrix)                                                          \
+        {                                                                     \
+            const FP* J = jones + 2 * (size_t) num_sources * s;               \
+            for (i = 0; i < num_sources; ++i)                                 \
+                sum[0] += (J[2 * i] * J[2 * i] + J[2 * i + 1] *              \
+                        J[2 * i + 1]) * I[i];                                 \
+            vis[2 * ((size_t) offset_out + s)] += sum[0];                     \
+            continue;                                                         \
+        }                                                                     \
+        for (i = 0; i < num_sources; ++i)                                     \
+        {                                                                     \
+            const FP* A = jones + 8 * ((size_t) num_sources * s + i);         \
+            const FP ba = I[i] + Q[i], bd = I[i] - Q[i];                      \
+            const FP bu = U[i], bv = V[i];                                    \
+            FP M[8];                                                          \
+            M[0] = A[0] * ba + (A[2] * bu + A[3] * bv);                       \
+            M[1] = A[1] * ba + (A[3] * bu - A[2] * bv);                       \
+            M[2] = A[2] * bd + (A[0] * bu - A[1] * bv);                       \
+            M[3] = A[3] * bd + (A[0] * bv + A[1] * bu);                       \
+            M[4] = A[4] * ba + (A[6] * bu + A[7] * bv);                       \
+            M[5] = A[5] * ba + (A[7] * bu - A[6] * bv);                       \
+            M[6] = A[6] * bd + (A[4] * bu - A[5] * bv);                       \
+            M[7] = A[7] * bd + (A[4] * bv + A[5] * bu);                       \
+            sum[0] += (M[0]*A[0] + M[1]*A[1]) + (M[2]*A[2] + M[3]*A[3]);      \
+            sum[1] += (M[1]*A[0] - M[0]*A[1]) + (M[3]*A[2] - M[2]*A[3]);      \
+            sum[2] += (M[0]*A[4] + M[1]*A[5]) + (M[2]*A[6] + M[3]*A[7]);      \
+            sum[3] += (M[1]*A[4] - M[0]*A[5]) + (M[3]*A[6] - M[2]*A[7]);      \
+            sum[4] += (M[4]*A[0] + M[5]*A[1]) + (M[6]*A[2] + M[7]*A[3]);      \
+            sum[5] += (M[5]*A[0] - M[4]*A[1]) + (M[7]*A[2] - M[6]*A[3]);      \
+            sum[6] += (M[4]*A[4] + M[5]*A[5]) + (M[6]*A[6] + M[7]*A[7]);      \
+            sum[7] += (M[5]*A[4] - M[4]*A[5]) + (M[7]*A[6] - M[6]*A[7]);      \
+        }                                                                     \
+        sum[1] = 0; sum[7] = 0;                                               \
+        for (r = 0; r < 8; ++r)                                               \
+            vis[8 * ((size_t) offset_out + s) + r] += sum[r];                 \
+    }                                                                         \
+}
+
+ACORR_BODY(oracle_auto_correlate_d, double)
+ACORR_BODY(oracle_auto_correlate_f, float)
+
 /* define_evaluate_jones_K.h:56-75 (CPU body) */
 void oracle_evaluate_jones_K_d(int num_sources, const double* l,
         const double* m, const double* n, int num_stations, const double* u,

@@ -48,6 +48,17 @@ void oracle_cross_correlate_f(int use_casa, int source_type, int matrix,
         int uv_units_wavelengths, double gast, double frequency_hz,
         int offset_out, float* vis);
 
+/* Auto-correlation, oskar/correlate/define_auto_correlate.h:32-58 and
+ * :84-104 (CPU bodies): matrix adds sum_i J_s(i) B_i J_s(i)^H to
+ * vis[offset_out + s] with zero imaginary parts on the diagonal; scalar adds
+ * sum_i |J_s(i)|^2 I_i to the real part only.  Plain sums in FP. */
+void oracle_auto_correlate_d(int matrix, int num_sources, int num_stations,
+        int offset_out, const double* jones, const double* I, const double* Q,
+        const double* U, const double* V, double* vis);
+void oracle_auto_correlate_f(int matrix, int num_sources, int num_stations,
+        int offset_out, const float* jones, const float* I, const float* Q,
+        const float* U, const float* V, float* vis);
+
 /* Weighted DFT, oskar/math/define_dftw_c2c.h:79-109 and
  * define_dftw_m2m.h:109-167 (CPU bodies).  z_in/z_out NULL -> 2D; data_idx
  * NULL -> identity. */

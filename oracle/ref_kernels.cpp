@@ -15,6 +15,8 @@
 #include "telescope/station/element/define_evaluate_dipole_pattern.h"
 #include "convert/define_convert_enu_directions_to_theta_phi.h"
 #include "interferometer/define_evaluate_jones_K.h"   /* :56-75 */
+#include "correlate/define_correlate_utils.h"
+#include "correlate/define_auto_correlate.h"        /* :32-58, :84-104 */
 
 #undef KERNEL
 #define KERNEL(NAME) extern "C" OSKAR_EXPORT void NAME
@@ -35,3 +37,7 @@ OSKAR_EVALUATE_DIPOLE_PATTERN_SCALAR(ref_dipole_pattern_scalar_double,
         double, double2)
 OSKAR_CONVERT_ENU_DIR_TO_THETA_PHI(ref_theta_phi_double, double)
 OSKAR_JONES_K_CPU(ref_jones_K_double, double, double2)
+OSKAR_ACORR_CPU(ref_acorr_float, float, float2, float4c)
+OSKAR_ACORR_CPU(ref_acorr_double, double, double2, double4c)
+OSKAR_ACORR_SCALAR_CPU(ref_acorr_scalar_float, float, float2)
+OSKAR_ACORR_SCALAR_CPU(ref_acorr_scalar_double, double, double2)

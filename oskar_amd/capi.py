@@ -136,6 +136,8 @@ SIGNATURES = {
     "oskar_cross_correlate_scalar_point_hip_d": (None, _XS_D),
     "oskar_cross_correlate_scalar_gaussian_hip_f": (None, _XSG_F),
     "oskar_cross_correlate_scalar_gaussian_hip_d": (None, _XSG_D),
+    # oskar_auto_correlate.h
+    "oskar_auto_correlate": (None, [I, P, MEMA, I, P, IP]),
     # oskar_cross_correlate_factorised.h
     "oskar_cross_correlate_factorised": (None, [I, I, P, MEMA, MEM3, P, MEM3,
                                                 D, I, I, P, IP]),

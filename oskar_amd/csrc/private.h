@@ -151,6 +151,12 @@ int oskar_amd_xcorr_launch(int precision, int matrix,
         const void* st_u, const void* st_v, const void* st_w,
         const void* st_x, const void* st_y, void* vis, void* stream);
 
+/* acorr.hip */
+int oskar_amd_acorr_launch(int precision, int matrix, int num_sources,
+        int num_stations, int offset_out, const void* jones, const void* I,
+        const void* Q, const void* U, const void* V, void* vis,
+        void* stream);
+
 /* dftw.hip */
 int oskar_amd_dftw_launch(int precision, int matrix, int is_3d,
         int num_in, double wavenumber, const void* weights,

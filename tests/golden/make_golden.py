@@ -22,6 +22,12 @@ in the precision of the kernel that produced them):
                      for {c2c, m2m} x {2D, 3D} x {float, double} x
                      {no data index, two element types}.
   beam_parts.npz     DFT weights, theta/phi, dipole patterns, Jones K.
+  acorr.npz          auto-correlation CPU bodies (define_auto_correlate.h)
+                     at the reference test's shapes and value ranges
+                     (oskar/correlate/test/Test_auto_correlate.cpp:40-63):
+                     50 stations, 277 sources, {scalar, matrix} x
+                     {float, double}, added at offset 3 into random vis
+                     (inputs stored as float32, exact in both precisions).
 """
 import ctypes as C
 import os
@@ -287,6 +293,35 @@ def make_beam_parts(ref, path, seed):
     np.savez_compressed(path, **out)
 
 
+def make_acorr(ref, path, seed):
+    rng = np.random.default_rng(seed)
+    ns, nsrc, off = 50, 277, 3
+    out = dict(num_stations=ns, num_sources=nsrc, offset_out=off)
+    f32 = np.float32
+    out["I"] = rng.uniform(1.0, 2.0, nsrc).astype(f32)
+    out["Q"] = rng.uniform(0.1, 1.0, nsrc).astype(f32)
+    out["U"] = rng.uniform(0.1, 0.5, nsrc).astype(f32)
+    out["V"] = rng.uniform(0.1, 0.2, nsrc).astype(f32)
+    for matrix in (0, 1):
+        nr = 8 if matrix else 2
+        jones = rng.uniform(1.0, 5.0, ns * nsrc * nr).astype(f32)
+        vis0 = rng.uniform(-1.0, 1.0, (off + ns + 2) * nr).astype(f32)
+        out["jones_%d" % matrix] = jones
+        out["vis0_%d" % matrix] = vis0
+        for prec in ("f", "d"):
+            dt = np.float32 if prec == "f" else np.float64
+            name = "ref_acorr_%s%s" % ("" if matrix else "scalar_",
+                                       "float" if prec == "f" else "double")
+            fn = getattr(ref, name)
+            fn.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6
+            fl = [np.ascontiguousarray(out[k], dt) for k in "IQUV"]
+            j = np.ascontiguousarray(jones, dt)
+            vis = np.array(vis0, dtype=dt, copy=True)
+            fn(nsrc, ns, off, *[P(a) for a in fl], P(j), P(vis))
+            out["vis_%d_%s" % (matrix, prec)] = vis
+    np.savez_compressed(path, **out)
+
+
 def main():
     ref = oracle.ref()
     if ref is None:
@@ -296,6 +331,7 @@ def main():
     make_xcorr(ref, os.path.join(HERE, "xcorr_long.npz"), 3, 13, 199, True)
     make_dftw(ref, os.path.join(HERE, "dftw.npz"), 4)
     make_beam_parts(ref, os.path.join(HERE, "beam_parts.npz"), 5)
+    make_acorr(ref, os.path.join(HERE, "acorr.npz"), 6)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -129,6 +129,37 @@ def test_cross_correlate_error_contract_on_host_memory():
     assert s.value == -9
 
 
+def test_auto_correlate_error_contract_on_host_memory():
+    """oskar_auto_correlate.c:30-46 order: location, type, dimension."""
+    jones, tel, flux, dirs, ext, uvw, vis = make_xcorr_args(OSKAR_CPU)
+    s2 = st()
+    vis_a = lib.oskar_mem_create(OSKAR_DOUBLE | OSKAR_COMPLEX | OSKAR_MATRIX,
+                                 OSKAR_CPU, 5, C.byref(s2))
+
+    def call(nsrc, v, status):
+        lib.oskar_auto_correlate(nsrc, jones, capi.MEMA(*flux), 0, v,
+                                 C.byref(status))
+
+    s = C.c_int(-3)
+    call(7, vis_a, s)
+    assert s.value == -3
+    s = st()
+    call(7, vis_a, s)
+    assert s.value == -14        # host memory: no CPU path in this library
+    vis_s = lib.oskar_mem_create(OSKAR_DOUBLE | OSKAR_COMPLEX, OSKAR_CPU, 5,
+                                 C.byref(s2))
+    s = st()
+    call(7, vis_s, s)
+    assert s.value == -9
+    s = st()
+    call(8, vis_a, s)
+    assert s.value == -11
+    # Location is checked before type: a CPU Jones with a (CPU) vis of the
+    # wrong type reports the type; a GPU vis would report the location, but
+    # needs a device.
+    assert s2.value == 0
+
+
 def test_dftw_error_contract_on_host_memory():
     s = st()
     f8, c16 = OSKAR_DOUBLE, OSKAR_DOUBLE | OSKAR_COMPLEX
