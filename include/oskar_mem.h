@@ -69,6 +69,19 @@ OSKAR_EXPORT
 oskar_Mem* oskar_mem_create_alias_from_raw(void* ptr, int type, int location,
         size_t num_elements, int* status);
 
+/* oskar/mem/oskar_mem_create_alias.h: a view of num_elements elements of
+ * src from offset (src NULL: an empty view for oskar_mem_set_alias). */
+OSKAR_EXPORT
+oskar_Mem* oskar_mem_create_alias(const oskar_Mem* src, size_t offset,
+        size_t num_elements, int* status);
+
+/* oskar/mem/oskar_mem_set_alias.h: re-points a view (not an owner:
+ * OSKAR_ERR_MEMORY_NOT_ALLOCATED) at src; OSKAR_ERR_OUT_OF_RANGE if the
+ * range passes the end of src. */
+OSKAR_EXPORT
+void oskar_mem_set_alias(oskar_Mem* mem, const oskar_Mem* src, size_t offset,
+        size_t num_elements, int* status);
+
 /* oskar/mem/oskar_mem_create_copy.h */
 OSKAR_EXPORT
 oskar_Mem* oskar_mem_create_copy(const oskar_Mem* src, int location,

@@ -181,6 +181,14 @@ oskar_StationWork* oskar_station_work_create(int type, int location,
         int* status);
 OSKAR_EXPORT void oskar_station_work_free(oskar_StationWork* work,
         int* status);
+/* Integer work arrays of the horizon clip
+ * (oskar/telescope/station/src/oskar_station_work.c:110-118):
+ * the mask of sources above the horizon of any station model, and its
+ * exclusive prefix sum with the total after the last source. */
+OSKAR_EXPORT oskar_Mem* oskar_station_work_horizon_mask(
+        oskar_StationWork* work);
+OSKAR_EXPORT oskar_Mem* oskar_station_work_source_indices(
+        oskar_StationWork* work);
 
 /* ---- Station-beam evaluation ------------------------------------------- */
 /* oskar/telescope/station/oskar_station_beam_horizon_direction.h:

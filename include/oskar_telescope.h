@@ -5,7 +5,7 @@
  * Reference: struct oskar/telescope/private_telescope.h:13-68 (fields
  * phase_centre_coord_type, phase_centre_rad, channel_bandwidth_hz :28,
  * time_average_sec :29, uv_filter_* :30-32, num_stations,
- * station_true_offset_ecef_metres :58); accessors
+ * station_true_offset_ecef_metres :58, station models :54-55); accessors
  * oskar/telescope/oskar_telescope_accessors.h.  Defaults follow
  * oskar/telescope/src/oskar_telescope_create.c (uv_filter_max = FLT_MAX,
  * units metres, phase centre type 0).
@@ -23,6 +23,10 @@ struct oskar_Telescope;
 #ifndef OSKAR_TELESCOPE_TYPEDEF_
 #define OSKAR_TELESCOPE_TYPEDEF_
 typedef struct oskar_Telescope oskar_Telescope;
+#endif
+#ifndef OSKAR_STATION_TYPEDEF_
+#define OSKAR_STATION_TYPEDEF_
+typedef struct oskar_Station oskar_Station;
 #endif
 
 OSKAR_EXPORT
@@ -61,6 +65,19 @@ OSKAR_EXPORT void oskar_telescope_set_time_average(oskar_Telescope* model,
         double time_average_sec);
 OSKAR_EXPORT void oskar_telescope_set_phase_centre(oskar_Telescope* model,
         int coord_type, double longitude_rad, double latitude_rad);
+/* Station models (oskar/telescope/src/oskar_telescope_resize_station_array.c,
+ * oskar_telescope_accessors.c:132-185): resizing creates empty models in
+ * the telescope's precision and location, or frees the surplus ones;
+ * oskar_telescope_station() returns NULL past the last model.  The
+ * horizon clip reads each model's longitude and latitude. */
+OSKAR_EXPORT void oskar_telescope_resize_station_array(
+        oskar_Telescope* model, int size, int* status);
+OSKAR_EXPORT int oskar_telescope_num_station_models(
+        const oskar_Telescope* model);
+OSKAR_EXPORT oskar_Station* oskar_telescope_station(oskar_Telescope* model,
+        int i);
+OSKAR_EXPORT const oskar_Station* oskar_telescope_station_const(
+        const oskar_Telescope* model, int i);
 /* units: "Metres" or "Wavelengths" (case-insensitive on the first letter),
  * oskar/telescope/src/oskar_telescope_accessors.c. */
 OSKAR_EXPORT void oskar_telescope_set_uv_filter(oskar_Telescope* model,

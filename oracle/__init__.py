@@ -47,6 +47,14 @@ def lib():
             [I, P, P, P, I, P, P, P, D, P, D, D, P])
         _lib.oracle_factorised_subset_ld.argtypes = (
             [I, P, P, P, P, P, I, P, P, P, D, I, P, P, P])
+        _lib.oracle_update_horizon_mask_d.argtypes = [I, P, P, P, D, D, D, P]
+        _lib.oracle_update_horizon_mask_f.argtypes = [I, P, P, P, D, D, D, P]
+        _lib.oracle_prefix_sum_int.argtypes = [I, P, P]
+        sc = [I, I, I, P, I, P, P, I, P, P]
+        _lib.oracle_sky_copy_source_data_d.argtypes = sc
+        _lib.oracle_sky_copy_source_data_f.argtypes = sc
+        _lib.oracle_sky_copy_source_data_d.restype = I
+        _lib.oracle_sky_copy_source_data_f.restype = I
         ac = [I, I, I, I] + [P] * 6
         _lib.oracle_auto_correlate_d.argtypes = ac
         _lib.oracle_auto_correlate_f.argtypes = ac
@@ -119,6 +127,50 @@ def auto_correlate(precision, matrix, num_stations, jones, I_, Q=None,
     fn(int(matrix), int(num_sources), int(num_stations), int(offset_out),
        *[_p(x) for x in arrs], _p(vis))
     return vis
+
+
+def update_horizon_mask(precision, l, m, n, ha0, dec0, lat, mask=None):
+    """mask |= (l ll + m mm + n nn > 0), oskar_update_horizon_mask
+    semantics; returns the int32 mask."""
+    dt = np.float64 if precision == "double" else np.float32
+    arrs = [_c(x, dt) for x in (l, m, n)]
+    num = arrs[0].size
+    mask = (np.zeros(num, np.int32) if mask is None
+            else np.ascontiguousarray(mask, np.int32).copy())
+    fn = (lib().oracle_update_horizon_mask_d if precision == "double"
+          else lib().oracle_update_horizon_mask_f)
+    fn(num, *[_p(x) for x in arrs], float(ha0), float(dec0), float(lat),
+       _p(mask))
+    return mask
+
+
+def prefix_sum(values):
+    """Exclusive prefix sum of int32 values with the total appended."""
+    v = np.ascontiguousarray(values, np.int32)
+    out = np.zeros(v.size + 1, np.int32)
+    lib().oracle_prefix_sum_int(v.size, _p(v), _p(out))
+    return out
+
+
+def sky_copy_source_data(precision, mask, table_in, cap_in, cap_out,
+                         valid_in, nfix, table_out=None, valid_out=None):
+    """Order-keeping masked row copy of a column table (num_col columns of
+    cap_in values) and its per-row valid-column counts; returns (rows
+    copied, table_out, valid_out)."""
+    dt = np.float64 if precision == "double" else np.float32
+    mask = np.ascontiguousarray(mask, np.int32)
+    tin = np.ascontiguousarray(table_in, dt)
+    num_col = tin.size // cap_in
+    vin = np.ascontiguousarray(valid_in, np.int32)
+    tout = (np.zeros(num_col * cap_out, dt) if table_out is None
+            else np.ascontiguousarray(table_out, dt).copy())
+    vout = (np.zeros(cap_out * nfix, np.int32) if valid_out is None
+            else np.ascontiguousarray(valid_out, np.int32).copy())
+    fn = (lib().oracle_sky_copy_source_data_d if precision == "double"
+          else lib().oracle_sky_copy_source_data_f)
+    r = fn(mask.size, cap_in, cap_out, _p(mask), num_col, _p(tin), _p(tout),
+           nfix, _p(vin), _p(vout))
+    return r, tout, vout
 
 
 def dftw(precision, matrix, normalise, wavenumber, weights, x_in, y_in, z_in,

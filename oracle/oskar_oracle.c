@@ -597,6 +597,65 @@ void NAME(int matrix, int num_sources, int num_stations, int offset_out,     \
 ACORR_BODY(oracle_auto_correlate_d, double)
 ACORR_BODY(oracle_auto_correlate_f, float)
 
+/*
+ * Horizon mask (oskar/sky/src/oskar_update_horizon_mask.c:30-60 and
+ * define_update_horizon_mask.h:3-12): (ll, mm, nn) in double, rounded to
+ * float for the float body; mask[i] |= l ll + m mm + n nn > 0.
+ */
+static void horizon_factors_d(double ha0, double dec0, double lat, double* f)
+{
+    const double cos_ha0 = cos(ha0), sin_dec0 = sin(dec0),
+            cos_dec0 = cos(dec0), sin_lat = sin(lat), cos_lat = cos(lat);
+    f[0] = cos_lat * sin(ha0);
+    f[1] = sin_lat * cos_dec0 - cos_lat * cos_ha0 * sin_dec0;
+    f[2] = sin_lat * sin_dec0 + cos_lat * cos_ha0 * cos_dec0;
+}
+
+void oracle_update_horizon_mask_d(int num, const double* l, const double* m,
+        const double* n, double ha0, double dec0, double lat, int* mask)
+{
+    double f[3];
+    int i;
+    horizon_factors_d(ha0, dec0, lat, f);
+    for (i = 0; i < num; ++i)
+        mask[i] |= ((l[i] * f[0] + m[i] * f[1] + n[i] * f[2]) > 0.0);
+}
+
+void oracle_update_horizon_mask_f(int num, const float* l, const float* m,
+        const float* n, double ha0, double dec0, double lat, int* mask)
+{
+    double f[3];
+    int i;
+    horizon_factors_d(ha0, dec0, lat, f);
+    {
+        const float a = (float) f[0], b = (float) f[1], c = (float) f[2];
+        for (i = 0; i < num; ++i)
+            mask[i] |= ((l[i] * a + m[i] * b + n[i] * c) > 0.0f);
+    }
+}
+
+/* Exclusive prefix sum with the total after the last value
+ * (oskar/math/src/oskar_prefix_sum.c:45-57). */
+void oracle_prefix_sum_int(int num, const int* in, int* out)
+{
+    int i, sum = 0;
+    for (i = 0; i < num; ++i)
+    {
+        const int x = in[i];
+        out[i] = sum;
+        sum += x;
+    }
+    out[num] = sum;
+}
+
+/* Masked row copy of the sky table (the CPU branch of
+ * oskar/sky/src/oskar_sky_copy_source_data.c:52-109): rows with mask != 0
+ * go, in order, to the next output row; nfix valid-column counts per row.
+ * Returns the number of rows copied. */
+#define SKY_COPY_BODY(NAME, FP)                                               int NAME(int num, int cap_in, int cap_out, const int* mask, int num_col,              const FP* table_in, FP* table_out, int nfix, const int* valid_in,             int* valid_out)                                                       {                                                                                 int i, c, r = 0;                                                              for (i = 0; i < num; ++i)                                                     {                                                                                 if (!mask[i]) continue;                                                       for (c = 0; c < num_col; ++c)                                                     table_out[(size_t) c * cap_out + r] =                                                 table_in[(size_t) c * cap_in + i];                                memcpy(valid_out + (size_t) nfix * r, valid_in + (size_t) nfix * i,                  (size_t) nfix * sizeof(int));                                         r++;                                                                      }                                                                             return r;                                                                 }
+SKY_COPY_BODY(oracle_sky_copy_source_data_d, double)
+SKY_COPY_BODY(oracle_sky_copy_source_data_f, float)
+
 /* define_evaluate_jones_K.h:56-75 (CPU body) */
 void oracle_evaluate_jones_K_d(int num_sources, const double* l,
         const double* m, const double* n, int num_stations, const double* u,

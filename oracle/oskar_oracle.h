@@ -59,6 +59,28 @@ void oracle_auto_correlate_f(int matrix, int num_sources, int num_stations,
         int offset_out, const float* jones, const float* I, const float* Q,
         const float* U, const float* V, float* vis);
 
+/* Horizon mask: mask[i] |= (l ll + m mm + n nn > 0) with (ll, mm, nn) of
+ * oskar/sky/src/oskar_update_horizon_mask.c:37-44 (float body: the factors
+ * rounded to float, as there). */
+void oracle_update_horizon_mask_d(int num, const double* l, const double* m,
+        const double* n, double ha0, double dec0, double lat, int* mask);
+void oracle_update_horizon_mask_f(int num, const float* l, const float* m,
+        const float* n, double ha0, double dec0, double lat, int* mask);
+
+/* Exclusive prefix sum, out[num] = total (oskar/math/src/oskar_prefix_sum.c). */
+void oracle_prefix_sum_int(int num, const int* in, int* out);
+
+/* Masked, order-keeping row copy of a column table (column c at
+ * c x capacity) with nfix valid-column counts per row
+ * (oskar/sky/src/oskar_sky_copy_source_data.c, CPU branch); returns the
+ * number of rows copied. */
+int oracle_sky_copy_source_data_d(int num, int cap_in, int cap_out,
+        const int* mask, int num_col, const double* table_in,
+        double* table_out, int nfix, const int* valid_in, int* valid_out);
+int oracle_sky_copy_source_data_f(int num, int cap_in, int cap_out,
+        const int* mask, int num_col, const float* table_in,
+        float* table_out, int nfix, const int* valid_in, int* valid_out);
+
 /* Weighted DFT, oskar/math/define_dftw_c2c.h:79-109 and
  * define_dftw_m2m.h:109-167 (CPU bodies).  z_in/z_out NULL -> 2D; data_idx
  * NULL -> identity. */

@@ -17,6 +17,9 @@
 #include "interferometer/define_evaluate_jones_K.h"   /* :56-75 */
 #include "correlate/define_correlate_utils.h"
 #include "correlate/define_auto_correlate.h"        /* :32-58, :84-104 */
+#include "sky/oskar_sky_enum.h"
+#include "sky/define_update_horizon_mask.h"          /* :3-12 */
+#include "sky/define_sky_copy_source_data.h"         /* :3-25 */
 
 #undef KERNEL
 #define KERNEL(NAME) extern "C" OSKAR_EXPORT void NAME
@@ -41,3 +44,7 @@ OSKAR_ACORR_CPU(ref_acorr_float, float, float2, float4c)
 OSKAR_ACORR_CPU(ref_acorr_double, double, double2, double4c)
 OSKAR_ACORR_SCALAR_CPU(ref_acorr_scalar_float, float, float2)
 OSKAR_ACORR_SCALAR_CPU(ref_acorr_scalar_double, double, double2)
+OSKAR_UPDATE_HORIZON_MASK(ref_update_horizon_mask_float, float)
+OSKAR_UPDATE_HORIZON_MASK(ref_update_horizon_mask_double, double)
+OSKAR_SKY_COPY_SOURCE_DATA(ref_sky_copy_source_data_float, float)
+OSKAR_SKY_COPY_SOURCE_DATA(ref_sky_copy_source_data_double, double)

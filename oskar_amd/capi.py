@@ -44,6 +44,8 @@ SIGNATURES = {
     "oskar_type_is_scalar": (I, [I]),
     "oskar_mem_element_size": (SZ, [I]),
     "oskar_mem_create": (P, [I, I, SZ, IP]),
+    "oskar_mem_create_alias": (P, [P, SZ, SZ, IP]),
+    "oskar_mem_set_alias": (None, [P, P, SZ, SZ, IP]),
     "oskar_mem_create_alias_from_raw": (P, [P, I, I, SZ, IP]),
     "oskar_mem_create_copy": (P, [P, I, IP]),
     "oskar_mem_free": (None, [P, IP]),
@@ -106,6 +108,10 @@ SIGNATURES = {
                                       IP]),
     # oskar_telescope.h
     "oskar_telescope_create": (P, [I, I, I, IP]),
+    "oskar_telescope_resize_station_array": (None, [P, I, IP]),
+    "oskar_telescope_num_station_models": (I, [P]),
+    "oskar_telescope_station": (P, [P, I]),
+    "oskar_telescope_station_const": (P, [P, I]),
     "oskar_telescope_free": (None, [P, IP]),
     "oskar_telescope_precision": (I, [P]),
     "oskar_telescope_mem_location": (I, [P]),
@@ -255,6 +261,26 @@ SIGNATURES = {
     "oskar_element_dipole_length": (D, [P]),
     "oskar_element_dipole_length_units": (I, [P]),
     "oskar_station_work_create": (P, [I, I, IP]),
+    "oskar_station_work_horizon_mask": (P, [P]),
+    "oskar_station_work_source_indices": (P, [P]),
+    # oskar_sky.h
+    "oskar_sky_create": (P, [I, I, I, IP]),
+    "oskar_sky_free": (None, [P, IP]),
+    "oskar_sky_double": (D, [P, I]),
+    "oskar_sky_int": (I, [P, I]),
+    "oskar_sky_set_double": (None, [P, I, D]),
+    "oskar_sky_set_int": (None, [P, I, I]),
+    "oskar_sky_column_attribute": (I, [P, I]),
+    "oskar_sky_column_type": (I, [P, I]),
+    "oskar_sky_num_valid_columns_of_type": (I, [P, I, I]),
+    "oskar_sky_set_num_valid_columns_of_type": (None, [P, I, I, I]),
+    "oskar_sky_column": (P, [P, I, I, IP]),
+    "oskar_sky_column_const": (P, [P, I, I]),
+    "oskar_sky_resize": (None, [P, I, IP]),
+    "oskar_update_horizon_mask": (None, [I, P, P, P, D, D, D, P, IP]),
+    "oskar_prefix_sum": (None, [SZ, P, P, IP]),
+    "oskar_sky_copy_source_data": (None, [P, P, P, P, IP]),
+    "oskar_sky_horizon_clip": (None, [P, P, P, D, P, IP]),
     "oskar_station_work_free": (None, [P, IP]),
     "oskar_station_beam_horizon_direction": (None, [P, D, DP, DP, DP, IP]),
     "oskar_evaluate_station_beam_aperture_array": (None, [P, P, I, P, P, P, I,

@@ -235,8 +235,59 @@ void oskar_telescope_free(oskar_Telescope* t, int* status)
     if (!t) return;
     for (i = 0; i < 3; ++i)
         oskar_mem_free(t->station_true_offset_ecef_metres[i], status);
+    for (i = 0; i < t->num_station_models; ++i)
+        oskar_station_free(t->station[i], status);
+    free(t->station);
     free(t);
 }
+
+/* oskar/telescope/src/oskar_telescope_resize_station_array.c: new station
+ * models are created empty, in the telescope's precision and location;
+ * shrinking frees the surplus ones. */
+void oskar_telescope_resize_station_array(oskar_Telescope* t, int size,
+        int* status)
+{
+    int i = 0;
+    if (*status) return;
+    if (size < 0)
+    {
+        *status = OSKAR_ERR_INVALID_ARGUMENT;
+        return;
+    }
+    if (size == t->num_station_models) return;
+    for (i = size; i < t->num_station_models; ++i)
+        oskar_station_free(t->station[i], status);
+    if (size > 0)
+    {
+        oskar_Station** s = (oskar_Station**) realloc(t->station,
+                (size_t) size * sizeof(oskar_Station*));
+        if (!s)
+        {
+            *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE;
+            return;
+        }
+        t->station = s;
+        for (i = t->num_station_models; i < size; ++i)
+            t->station[i] = oskar_station_create(t->precision,
+                    t->mem_location, 0, status);
+    }
+    else
+    {
+        free(t->station);
+        t->station = 0;
+    }
+    t->num_station_models = size;
+}
+
+int oskar_telescope_num_station_models(const oskar_Telescope* t)
+{ return t->num_station_models; }
+
+oskar_Station* oskar_telescope_station(oskar_Telescope* t, int i)
+{ return (i >= 0 && i < t->num_station_models) ? t->station[i] : 0; }
+
+const oskar_Station* oskar_telescope_station_const(const oskar_Telescope* t,
+        int i)
+{ return (i >= 0 && i < t->num_station_models) ? t->station[i] : 0; }
 
 int oskar_telescope_precision(const oskar_Telescope* m) { return m->precision; }
 int oskar_telescope_mem_location(const oskar_Telescope* m)

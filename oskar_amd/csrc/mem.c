@@ -146,6 +146,49 @@ oskar_Mem* oskar_mem_create_alias_from_raw(void* ptr, int type, int location,
     return mem;
 }
 
+oskar_Mem* oskar_mem_create_alias(const oskar_Mem* src, size_t offset,
+        size_t num_elements, int* status)
+{
+    oskar_Mem* mem = (oskar_Mem*) calloc(1, sizeof(oskar_Mem));
+    if (!mem)
+    {
+        *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE;
+        return 0;
+    }
+    mem->owner = 0;
+    mem->ref_count = 1;
+    if (src)
+    {
+        mem->type = src->type;
+        mem->location = src->location;
+        mem->num_elements = num_elements;
+        mem->data = src->data ? (char*) src->data +
+                offset * oskar_mem_element_size(src->type) : 0;
+    }
+    return mem;
+}
+
+void oskar_mem_set_alias(oskar_Mem* mem, const oskar_Mem* src, size_t offset,
+        size_t num_elements, int* status)
+{
+    if (*status) return;
+    if (mem->owner)
+    {
+        *status = OSKAR_ERR_MEMORY_NOT_ALLOCATED;
+        return;
+    }
+    if (offset + num_elements > src->num_elements)
+    {
+        *status = OSKAR_ERR_OUT_OF_RANGE;
+        return;
+    }
+    mem->type = src->type;
+    mem->location = src->location;
+    mem->num_elements = num_elements;
+    mem->data = src->data ? (char*) src->data +
+            offset * oskar_mem_element_size(src->type) : 0;
+}
+
 oskar_Mem* oskar_mem_create_copy(const oskar_Mem* src, int location,
         int* status)
 {

@@ -11,6 +11,7 @@
 #include "oskar_jones.h"
 #include "oskar_telescope.h"
 #include "oskar_station.h"
+#include "oskar_sky.h"
 #include "oskar_timer.h"
 
 #ifdef __cplusplus
@@ -54,6 +55,8 @@ struct oskar_Telescope
     double uv_filter_max;
     int uv_filter_units;
     oskar_Mem* station_true_offset_ecef_metres[3];
+    int num_station_models;      /* private_telescope.h:54 */
+    oskar_Station** station;     /* station models */
 };
 
 /* oskar/telescope/station/element/private_element.h (dipole/isotropic). */
@@ -107,6 +110,23 @@ struct oskar_StationWork
     oskar_Mem* phi_y;
     oskar_Mem* enu[3];        /* source directions (+1 normalisation point) */
     oskar_Mem* beam_out;      /* beam of those directions */
+    oskar_Mem* horizon_mask;  /* int, one per sky source */
+    oskar_Mem* source_indices;/* int, one per sky source + 1 (the total) */
+};
+
+/* oskar/sky/private_sky.h: one table of num_columns x capacity values
+ * (column c at offset c x capacity), per-row valid-column counts
+ * (capacity x OSKAR_SKY_NUM_FIXED_COLUMN_TYPES ints), column aliases,
+ * types and attributes. */
+struct oskar_Sky
+{
+    double attr_double[OSKAR_SKY_NUM_ATTRIBUTES_DOUBLE];
+    int attr_int[OSKAR_SKY_NUM_ATTRIBUTES_INT];
+    oskar_Mem* table;
+    oskar_Mem* num_valid_columns;
+    oskar_Mem** columns;
+    oskar_SkyColumn* column_type;
+    int* column_attr;
 };
 
 struct oskar_Timer
@@ -156,6 +176,33 @@ int oskar_amd_acorr_launch(int precision, int matrix, int num_sources,
         int num_stations, int offset_out, const void* jones, const void* I,
         const void* Q, const void* U, const void* V, void* vis,
         void* stream);
+
+/* sky_clip.hip */
+/* Up to SKY_CLIP_MAX_MODELS station models per mask launch: (ll, mm, nn)
+ * of each, in FP64 (rounded to the kernel precision on the device). */
+#define SKY_CLIP_MAX_MODELS 16
+typedef struct oskar_HorizonModels
+{
+    int num;
+    double lmn[SKY_CLIP_MAX_MODELS][3];
+} oskar_HorizonModels;
+int oskar_amd_horizon_mask_launch(int precision, int num,
+        const void* l, const void* m, const void* n,
+        const oskar_HorizonModels* models, int* mask, void* stream);
+int oskar_amd_prefix_sum_launch(int num, const int* in, int* out,
+        void* stream);
+int oskar_amd_sky_copy_launch(int precision, int num_in, int capacity_in,
+        int capacity_out, const int* mask, const int* indices,
+        int num_columns, const void* table_in, void* table_out,
+        const int* valid_in, int* valid_out, void* stream);
+/* Fused clip: mask over all models, block counts, block offsets, then the
+ * block scan writes indices (total at [num_in]) and scatters the rows. */
+int oskar_amd_horizon_clip_launch(int precision, int num_in,
+        const void* l, const void* m, const void* n,
+        const oskar_HorizonModels* models, int* mask, int* indices,
+        int capacity_in, int capacity_out, int num_columns,
+        const void* table_in, void* table_out, const int* valid_in,
+        int* valid_out, void* stream);
 
 /* dftw.hip */
 int oskar_amd_dftw_launch(int precision, int matrix, int is_3d,

@@ -480,8 +480,16 @@ oskar_StationWork* oskar_station_work_create(int type, int location,
     w->enu[0] = oskar_mem_create(type, location, 0, status);
     w->enu[1] = oskar_mem_create(type, location, 0, status);
     w->enu[2] = oskar_mem_create(type, location, 0, status);
+    w->horizon_mask = oskar_mem_create(OSKAR_INT, location, 0, status);
+    w->source_indices = oskar_mem_create(OSKAR_INT, location, 0, status);
     return w;
 }
+
+oskar_Mem* oskar_station_work_horizon_mask(oskar_StationWork* w)
+{ return w->horizon_mask; }
+
+oskar_Mem* oskar_station_work_source_indices(oskar_StationWork* w)
+{ return w->source_indices; }
 
 void oskar_station_work_free(oskar_StationWork* w, int* status)
 {
@@ -495,6 +503,8 @@ void oskar_station_work_free(oskar_StationWork* w, int* status)
     oskar_mem_free(w->enu[1], status);
     oskar_mem_free(w->enu[2], status);
     oskar_mem_free(w->beam_out, status);
+    oskar_mem_free(w->horizon_mask, status);
+    oskar_mem_free(w->source_indices, status);
     free(w);
 }
 

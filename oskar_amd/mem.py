@@ -1,5 +1,6 @@
 """Python handles over the C-ABI structures (oskar_Mem, oskar_Jones,
-oskar_Telescope, oskar_Station, oskar_StationWork, oskar_FactorisedBatch).
+oskar_Telescope, oskar_Station, oskar_StationWork, oskar_Sky,
+oskar_FactorisedBatch).
 
 Mirrors the way the reference's own tests drive the C API
 (oskar/correlate/test/Test_cross_correlate.cpp:60-120): create typed
@@ -236,6 +237,60 @@ class StationWork:
         try:
             if self.ptr:
                 lib.oskar_station_work_free(self.ptr, C.byref(C.c_int(0)))
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class Sky:
+    """Handle to an oskar_Sky column table (oskar/sky/private_sky.h)."""
+
+    # oskar_SkyAttribInt / oskar_SkyAttribDouble
+    PRECISION, MEM_LOCATION, CAPACITY, NUM_SOURCES, NUM_COLUMNS, \
+        USE_EXTENDED = range(6)
+    REF_RA_RAD, REF_DEC_RAD = 0, 1
+
+    def __init__(self, precision, location, num_sources):
+        st = C.c_int(0)
+        self.ptr = lib.oskar_sky_create(precision, location, num_sources,
+                                        C.byref(st))
+        check_status(st, "oskar_sky_create")
+
+    def attr(self, a):
+        return lib.oskar_sky_int(self.ptr, a)
+
+    def set_reference(self, ra_rad, dec_rad):
+        lib.oskar_sky_set_double(self.ptr, self.REF_RA_RAD, float(ra_rad))
+        lib.oskar_sky_set_double(self.ptr, self.REF_DEC_RAD, float(dec_rad))
+
+    def column(self, column_type, attribute=0):
+        """The column (created if missing) as a non-owning Mem view."""
+        st = C.c_int(0)
+        p = lib.oskar_sky_column(self.ptr, column_type, attribute,
+                                 C.byref(st))
+        check_status(st, "oskar_sky_column")
+        return _Borrowed(p)
+
+    def set_column(self, column_type, values, attribute=0):
+        """Writes values into rows [0, len(values)) of a column."""
+        col = self.column(column_type, attribute)
+        src = Mem.from_numpy(values, col.type, OSKAR_CPU)
+        st = C.c_int(0)
+        lib.oskar_mem_copy_contents(col.ptr, src.ptr, 0, 0, len(src),
+                                    C.byref(st))
+        check_status(st, "Sky.set_column")
+
+    def get_column(self, column_type, attribute=0):
+        """Rows [0, NUM_SOURCES) of an existing column (None if absent)."""
+        p = lib.oskar_sky_column_const(self.ptr, column_type, attribute)
+        if not p:
+            return None
+        return _Borrowed(p).numpy()[:self.attr(self.NUM_SOURCES)]
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.oskar_sky_free(self.ptr, C.byref(C.c_int(0)))
                 self.ptr = None
         except Exception:
             pass

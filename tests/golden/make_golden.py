@@ -28,6 +28,13 @@ in the precision of the kernel that produced them):
                      50 stations, 277 sources, {scalar, matrix} x
                      {float, double}, added at offset 3 into random vis
                      (inputs stored as float32, exact in both precisions).
+  sky_clip.npz       horizon mask bodies (define_update_horizon_mask.h) for
+                     three (hour angle, Dec, latitude) cases on 3000
+                     directions, 600 of them within ~1e-7 of the horizon
+                     plane, {float, double}, each case alone and ORed over
+                     the three; the masked row copy
+                     (define_sky_copy_source_data.h) of a 7-column table
+                     with its valid-column counts, {float, double}.
 """
 import ctypes as C
 import os
@@ -322,6 +329,77 @@ def make_acorr(ref, path, seed):
     np.savez_compressed(path, **out)
 
 
+def horizon_factors(ha0, dec0, lat):
+    """(ll, mm, nn) as oskar/sky/src/oskar_update_horizon_mask.c:37-44
+    computes them (libm, double)."""
+    import math
+    return (math.cos(lat) * math.sin(ha0),
+            math.sin(lat) * math.cos(dec0) -
+            math.cos(lat) * math.cos(ha0) * math.sin(dec0),
+            math.sin(lat) * math.sin(dec0) +
+            math.cos(lat) * math.cos(ha0) * math.cos(dec0))
+
+
+def make_sky_clip(ref, path, seed):
+    rng = np.random.default_rng(seed)
+    f32 = np.float32
+    cases = np.array([[0.7, -0.45, -0.47], [2.9, 0.2, 0.65],
+                      [-1.3, -1.1, -0.9]])
+    num = 3000
+    v = rng.normal(size=(num, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    # 200 directions near each case's horizon plane: project out the plane
+    # normal, renormalise, tilt by up to 1e-7.
+    for k, (ha0, dec0, lat) in enumerate(cases):
+        f = np.array(horizon_factors(ha0, dec0, lat))
+        w = v[k * 200:(k + 1) * 200]
+        w -= np.outer(w @ f, f)
+        w /= np.linalg.norm(w, axis=1)[:, None]
+        w += np.outer(rng.uniform(-1e-7, 1e-7, 200), f)
+        v[k * 200:(k + 1) * 200] = w
+    v = v.astype(f32)
+    out = dict(l=v[:, 0], m=v[:, 1], n=v[:, 2], cases=cases)
+    for prec in ("f", "d"):
+        dt = f32 if prec == "f" else np.float64
+        fn = getattr(ref, "ref_update_horizon_mask_%s" % (
+            "float" if prec == "f" else "double"))
+        ct = C.c_float if prec == "f" else C.c_double
+        fn.argtypes = [C.c_int] + [C.c_void_p] * 3 + [ct] * 3 + [C.c_void_p]
+        l, m, n = (np.ascontiguousarray(out[k], dt) for k in "lmn")
+        acc = np.zeros(num, np.int32)
+        for k, (ha0, dec0, lat) in enumerate(cases):
+            ff = horizon_factors(ha0, dec0, lat)
+            one = np.zeros(num, np.int32)
+            fn(num, P(l), P(m), P(n), *[ct(x) for x in ff], P(one))
+            fn(num, P(l), P(m), P(n), *[ct(x) for x in ff], P(acc))
+            out["mask_%s_%d" % (prec, k)] = one
+        out["mask_%s_or" % prec] = acc
+    # Row copy: 7 columns, input capacity 3072, output capacity 4096.
+    ncol, cap_in, cap_out, nfix = 7, 3072, 4096, 20
+    mask = out["mask_d_or"].copy()
+    mask[rng.integers(0, num, 300)] = 0
+    idx = np.zeros(num + 1, np.int32)
+    idx[1:] = np.cumsum(mask)
+    out.update(copy_mask=mask, copy_indices=idx, ncol=ncol, cap_in=cap_in,
+               cap_out=cap_out, nfix=nfix)
+    out["table_in"] = rng.uniform(-10, 10, ncol * cap_in).astype(f32)
+    out["valid_in"] = rng.integers(0, 4, cap_in * nfix).astype(np.int32)
+    for prec in ("f", "d"):
+        dt = f32 if prec == "f" else np.float64
+        fn = getattr(ref, "ref_sky_copy_source_data_%s" % (
+            "float" if prec == "f" else "double"))
+        fn.argtypes = [C.c_int] * 3 + [C.c_void_p] * 2 + [C.c_int] + \
+            [C.c_void_p] * 4
+        tin = np.ascontiguousarray(out["table_in"], dt)
+        tout = np.full(ncol * cap_out, -7.0, dt)
+        vout = np.full(cap_out * nfix, -1, np.int32)
+        fn(num, cap_in, cap_out, P(mask), P(idx), ncol, P(tin), P(tout),
+           P(out["valid_in"]), P(vout))
+        out["table_out_%s" % prec] = tout
+        out["valid_out_%s" % prec] = vout
+    np.savez_compressed(path, **out)
+
+
 def main():
     ref = oracle.ref()
     if ref is None:
@@ -332,6 +410,7 @@ def main():
     make_dftw(ref, os.path.join(HERE, "dftw.npz"), 4)
     make_beam_parts(ref, os.path.join(HERE, "beam_parts.npz"), 5)
     make_acorr(ref, os.path.join(HERE, "acorr.npz"), 6)
+    make_sky_clip(ref, os.path.join(HERE, "sky_clip.npz"), 7)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
