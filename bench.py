@@ -113,6 +113,9 @@ def main():
     ap.add_argument("--elements", type=int, default=256)
     ap.add_argument("--freq", type=float, default=150e6)
     ap.add_argument("--mode", choices=["fp16x3", "bf16x3"], default="fp16x3")
+    ap.add_argument("--tile-shape", type=int, default=-1,
+                    help="tile shape of the one-call factorised path "
+                         "(0-4; default: the library's)")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--auto-steps", type=int, default=5,
@@ -205,6 +208,10 @@ def main():
         return t
 
     # --- factorised pipeline -------------------------------------------
+    if args.tile_shape >= 0 and \
+            lib.oskar_cross_correlate_factorised_set_tile_shape(
+                args.tile_shape) < 0:
+        raise SystemExit("bad --tile-shape %d" % args.tile_shape)
     vis.clear()
     t_fact = timed(step_fact, args.steps, args.warmup, "factorised")
     fact_rate = evals * args.steps / t_fact
