@@ -506,6 +506,9 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
             wh[cf] = st[P_ENTRIES + (0 * 2 + lh) * TW + row];
             wl[cf] = st[P_ENTRIES + (1 * 2 + lh) * TW + row];
         }
+        /* Two workgroups per CU: raise this wave's priority while its
+         * MFMAs issue (see the schedule below). */
+        if constexpr (TL::OCC >= 2) __builtin_amdgcn_s_setprio(2);
         /* hi*hi, then hi*lo, then lo*hi. */
 #pragma unroll
         for (int rf = 0; rf < RF; ++rf)
@@ -531,22 +534,40 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
                 re[rf][cf] = mfma<T16>(xl[rf], wh[cf], re[rf][cf]);
                 im[rf][cf] = mfma<T16>(yl[rf], wh[cf], im[rf][cf]);
             }
+        if constexpr (TL::OCC >= 2) __builtin_amdgcn_s_setprio(0);
         /* Build step ks+1 (a masked dummy after the last step) into the
          * other stage, then start the loads for step ks+2. */
         bld.build(s_begin + (ks + 1) * KSRC, cur, lds[(ks + 1) & 1]);
         bld.load(J, s_begin + (ks + 2) * KSRC, &cur);
-        /* Schedule: operand reads, then each MFMA followed by a few VALU
-         * instructions of the build, then the builder's LDS writes and the
-         * next loads. */
-        __builtin_amdgcn_sched_group_barrier(0x100, 4 * RF + 2 * CF, 0);
-#pragma unroll
-        for (int i = 0; i < TL::NMFMA; ++i)
+        if constexpr (TL::OCC >= 2)
         {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 192 / TL::NMFMA, 0);
+            /* Two workgroups per CU (two waves per SIMD from independent
+             * workgroups): operand reads, all MFMAs, then the whole build,
+             * its LDS writes and the next loads.  The two workgroups drift
+             * into opposite phases, so one's MFMAs run beside the other's
+             * build (measured 6 % faster than interleaving at 64 x 64). */
+            __builtin_amdgcn_sched_group_barrier(0x100, 4 * RF + 2 * CF, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, TL::NMFMA, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 400, 0);
+            __builtin_amdgcn_sched_group_barrier(0x200, 12, 0);
+            __builtin_amdgcn_sched_group_barrier(0x020, 2 * PS + 3 * QS, 0);
         }
-        __builtin_amdgcn_sched_group_barrier(0x200, 12, 0);
-        __builtin_amdgcn_sched_group_barrier(0x020, 2 * PS + 3 * QS, 0);
+        else
+        {
+            /* One workgroup per CU: operand reads, then each MFMA followed
+             * by a few VALU instructions of the build, then the builder's
+             * LDS writes and the next loads. */
+            __builtin_amdgcn_sched_group_barrier(0x100, 4 * RF + 2 * CF, 0);
+#pragma unroll
+            for (int i = 0; i < TL::NMFMA; ++i)
+            {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002,
+                        192 / TL::NMFMA, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x200, 12, 0);
+            __builtin_amdgcn_sched_group_barrier(0x020, 2 * PS + 3 * QS, 0);
+        }
         __syncthreads();
     }
 #pragma unroll
