@@ -18,6 +18,8 @@ Measured:
   error    relative RMS of both against an FP64 / long-double CPU oracle on
            a random sample of baselines at the full source count;
   roofline the MFMA product kernel, timed with HIP events on its stream;
+  clip     oskar_sky_horizon_clip of a 1M-source all-sky catalogue (18
+           columns) at 10 sidereal times, against the 8 TB/s HBM peak;
   auto     oskar_auto_correlate on the plain pipeline's joined Jones
            (512 x 1M FP32 2x2 matrices, 16.4 GB), an HBM-bound sum,
            against the 8 TB/s HBM peak;
@@ -118,6 +120,8 @@ def main():
                          "(0-4; default: the library's)")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--clip-steps", type=int, default=10,
+                    help="horizon clips of a 1M-source catalogue (0 = skip)")
     ap.add_argument("--auto-steps", type=int, default=5,
                     help="oskar_auto_correlate steps on the plain "
                          "pipeline's Jones (0 = skip)")
@@ -366,6 +370,11 @@ def main():
                                     flux, dirs, uvw, beam, ref, bl, timed,
                                     evals)
 
+    # --- horizon clip of an all-sky catalogue --------------------------
+    clip = None
+    if args.clip_steps > 0:
+        clip = bench_clip(args, rng)
+
     # --- batched (channel, time) units in one persistent launch ---------
     batch = None
     if args.batch_units > 0:
@@ -456,8 +465,82 @@ def main():
         "batch": batch,
         "surrogate": surrogate,
         "auto_correlate": auto,
+        "horizon_clip": clip,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_clip(args, rng):
+    """oskar_sky_horizon_clip of a 1M-source all-sky catalogue (RA, Dec,
+    I, Q, U, V, reference frequency, spectral index, rotation measure,
+    major, minor, PA, l, m, n and the three Gaussian parameters: 18 FP32
+    columns) about a phase centre at Dec -30 deg, for one station model at
+    latitude -26.7 deg, at `clip_steps` sidereal times over a day.  Each
+    clip ends with the 4-byte read of the kept count (the host needs it).
+    Bytes per clip: l, m, n read and the mask written (16 B per source),
+    mask read and index written (8 B), and for each kept source its row
+    read and written (2 x 18 x 4 B) with its valid-column counts
+    (2 x 80 B)."""
+    from oskar_amd import lib
+    from oskar_amd.mem import (OSKAR_GPU, OSKAR_SINGLE, Sky, StationWork,
+                               Telescope, check_status)
+    S = 1_000_000
+    ra0, dec0 = 0.0, np.radians(-30.0)
+    ra = rng.uniform(0, 2 * np.pi, S)
+    dec = np.arcsin(rng.uniform(-1, 1, S))
+    d = ra - ra0
+    l = np.cos(dec) * np.sin(d)
+    m = np.cos(dec0) * np.sin(dec) - np.sin(dec0) * np.cos(dec) * np.cos(d)
+    n = np.sin(dec0) * np.sin(dec) + np.cos(dec0) * np.cos(dec) * np.cos(d)
+    cols = [(1, ra), (2, dec), (3, rng.uniform(0.01, 10, S))]
+    cols += [(t, rng.uniform(-0.1, 0.1, S)) for t in (4, 5, 6)]
+    cols += [(7, np.full(S, 1e8)), (8, np.full(S, -0.7)), (9, np.zeros(S))]
+    cols += [(t, np.zeros(S)) for t in (10, 11, 12)]
+    cols += [(0x103, l), (0x104, m), (0x105, n)]
+    cols += [(t, np.zeros(S)) for t in (0x100, 0x101, 0x102)]
+    sky_in = Sky(OSKAR_SINGLE, OSKAR_GPU, S)
+    for t, v in cols:
+        sky_in.set_column(t, v)
+    sky_in.set_reference(ra0, dec0)
+    sky_out = Sky(OSKAR_SINGLE, OSKAR_GPU, S)
+    tel = Telescope(OSKAR_SINGLE, OSKAR_GPU, 4)
+    work = StationWork(OSKAR_SINGLE, OSKAR_GPU)
+    status = C.c_int(0)
+    lib.oskar_telescope_resize_station_array(tel.ptr, 1, C.byref(status))
+    lib.oskar_station_set_position(lib.oskar_telescope_station(tel.ptr, 0),
+                                   np.radians(116.7), np.radians(-26.7), 0.0,
+                                   0.0, 0.0, 0.0)
+    gast = np.linspace(0, 2 * np.pi, args.clip_steps, endpoint=False)
+
+    def clip_once(g):
+        lib.oskar_sky_horizon_clip(sky_out.ptr, sky_in.ptr, tel.ptr, float(g),
+                                   work.ptr, C.byref(status))
+
+    clip_once(0.0)                          # warm-up (sizes the outputs)
+    check_status(status, "horizon clip")
+    kept = []
+    timer = lib.oskar_timer_create(1)
+    lib.oskar_timer_start(timer)
+    for g in gast:
+        clip_once(g)
+        kept.append(lib.oskar_sky_int(sky_out.ptr, Sky.NUM_SOURCES))
+    t = lib.oskar_timer_elapsed(timer)
+    lib.oskar_timer_free(timer)
+    check_status(status, "horizon clip")
+    ms = 1e3 * t / len(gast)
+    ncol = len(cols)
+    nbytes = [S * 24 + k * 2 * (ncol * 4 + 80) for k in kept]
+    gbps = float(np.mean(nbytes)) / (ms * 1e-3) / 1e9
+    log("horizon clip: %.3f ms per clip, kept %d-%d of %d, %.0f GB/s" % (
+        ms, min(kept), max(kept), S, gbps))
+    return {"workload": "1M-source all-sky catalogue, 18 FP32 columns, one "
+                        "station model, %d sidereal times" % len(gast),
+            "ms_per_clip": round(ms, 4),
+            "kept_min": int(min(kept)), "kept_max": int(max(kept)),
+            "algorithmic_bytes_mean": float(np.mean(nbytes)),
+            "achieved_GBps": round(gbps, 1),
+            "hbm_peak_GBps": PEAK_HBM_GBPS,
+            "frac": round(gbps / PEAK_HBM_GBPS, 4)}
 
 
 def bench_batch(args, st, station, work, mode):
