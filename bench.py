@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--auto-steps", type=int, default=5,
                     help="oskar_auto_correlate steps on the plain "
                          "pipeline's Jones (0 = skip)")
+    ap.add_argument("--vis-blocks", type=int, default=3,
+                    help="512-station vis blocks written and read back "
+                         "(0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--error-baselines", type=int, default=48)
@@ -371,6 +374,9 @@ def main():
                                     evals)
 
     # --- horizon clip of an all-sky catalogue --------------------------
+    vis = None
+    if args.vis_blocks > 0:
+        vis = bench_vis(args, rng)
     clip = None
     if args.clip_steps > 0:
         clip = bench_clip(args, rng)
@@ -466,6 +472,7 @@ def main():
         "surrogate": surrogate,
         "auto_correlate": auto,
         "horizon_clip": clip,
+        "vis_block": vis,
     }
     print(json.dumps(line), flush=True)
 
@@ -541,6 +548,122 @@ def bench_clip(args, rng):
             "achieved_GBps": round(gbps, 1),
             "hbm_peak_GBps": PEAK_HBM_GBPS,
             "frac": round(gbps / PEAK_HBM_GBPS, 4)}
+
+
+def bench_vis(args, rng):
+    """The correlator's output container at 512 stations: a device vis
+    block of 10 times x 8 channels of FP32 2x2 cross-correlations
+    (130 816 baselines, 335 MB) with auto-correlations and station
+    (u, v, w), written `vis_blocks` times to a .vis file under /tmp
+    (device-to-host through pinned staging, CRC-32C, file write), read back
+    into a device block (file read, CRC-32C, host-to-device, baseline
+    (u, v, w) from the station (u, v, w) by the HIP kernel), and the
+    station-to-baseline kernel alone, timed with HIP events on the library
+    stream.  Kernel bytes: 3 x 4 B read per station and 3 x 4 B written per
+    baseline, per time."""
+    import tempfile
+    from oskar_amd import lib
+    from oskar_amd.mem import OSKAR_GPU, OSKAR_SINGLE, Mem, check_status
+    from oskar_amd.vis import Binary, VisBlock, VisHeader
+    ns, nt, nc = 512, 10, 8
+    nb = ns * (ns - 1) // 2
+    amp = OSKAR_SINGLE | 0x20 | 0x40
+    hdr = VisHeader(amp, OSKAR_SINGLE, nt, nt * args.vis_blocks, nc, nc, ns,
+                    1, 1)
+    blk = VisBlock(OSKAR_GPU, header=hdr)
+    st = C.c_int(0)
+    for m, n in ((blk.cross(), nt * nc * nb * 8), (blk.auto(), nt * nc * ns
+                                                     * 8)):
+        src = Mem.from_numpy(rng.normal(size=n).astype(np.float32), amp)
+        lib.oskar_mem_copy(m.ptr, src.ptr, C.byref(st))
+        src.free()
+    for k in range(3):
+        src = Mem.from_numpy(rng.uniform(-5e4, 5e4, nt * ns).astype(
+            np.float32), OSKAR_SINGLE)
+        lib.oskar_mem_copy(blk.station_uvw(k).ptr, src.ptr, C.byref(st))
+        src.free()
+    check_status(st, "vis block fill")
+    block_bytes = (nt * nc * (nb + ns) * 32) + 3 * nt * ns * 4
+    fd, path = tempfile.mkstemp(suffix=".vis", dir="/tmp")
+    os.close(fd)
+    try:
+        h = hdr.write(path)
+        t0 = time.perf_counter()
+        for i in range(args.vis_blocks):
+            lib.oskar_vis_block_set_start_time_index(blk.ptr, nt * i)
+            blk.write(h, i)
+        h.close()
+        t_write = (time.perf_counter() - t0) / args.vis_blocks
+        b = Binary(path, "r")
+        got_hdr = VisHeader.read(b)
+        got = VisBlock(OSKAR_GPU, header=got_hdr)
+        got.read(got_hdr, b, 0)                 # warm-up (sizes arrays)
+        t0 = time.perf_counter()
+        for i in range(args.vis_blocks):
+            got.read(got_hdr, b, i)
+        lib.oskar_device_synchronize(OSKAR_GPU)
+        t_read = (time.perf_counter() - t0) / args.vis_blocks
+        b.close()
+        same = bool(np.array_equal(got.cross().numpy()[:4096],
+                                   blk.cross().numpy()[:4096]))
+    finally:
+        os.remove(path)
+    # The conversion kernel alone, HIP events on the library stream.
+    reps = 50
+    timer = lib.oskar_timer_create(1)
+    lib.oskar_vis_block_station_to_baseline_coords(blk.ptr, hdr.ptr,
+                                                   C.byref(st))
+    lib.oskar_timer_start(timer)
+    for _ in range(reps):
+        lib.oskar_vis_block_station_to_baseline_coords(blk.ptr, hdr.ptr,
+                                                       C.byref(st))
+    t_k = lib.oskar_timer_elapsed(timer) / reps
+    lib.oskar_timer_free(timer)
+    check_status(st, "station to baseline")
+    ref_gbps = None
+    if not args.no_cpu:
+        # The reference's binary writer (oskar/binary/src, built into
+        # oracle/_ref) on the same block bytes already in host memory.
+        import oracle
+        ref = oracle.ref_binary()
+        if ref is not None:
+            host = blk.cross().numpy()
+            fd, path = tempfile.mkstemp(suffix=".dat", dir="/tmp")
+            os.close(fd)
+            try:
+                rs = C.c_int(0)
+                t0 = time.perf_counter()
+                for i in range(args.vis_blocks):
+                    rh = ref.oskar_binary_create(path.encode(), b"w",
+                                                 C.byref(rs))
+                    ref.oskar_binary_write(rh, amp, 12, 3, i, host.nbytes,
+                                           host.ctypes.data, C.byref(rs))
+                    ref.oskar_binary_free(rh)
+                t_ref = (time.perf_counter() - t0) / args.vis_blocks
+                ref_gbps = round(host.nbytes / t_ref / 1e9, 2)
+            finally:
+                os.remove(path)
+            log("vis block: reference writer from host memory %.2f GB/s"
+                % ref_gbps)
+    k_bytes = nt * (3 * ns * 4 + 3 * nb * 4)
+    k_gbps = k_bytes / t_k / 1e9
+    w_gbps = block_bytes / t_write / 1e9
+    r_gbps = block_bytes / t_read / 1e9
+    log("vis block: write %.1f ms (%.2f GB/s), read %.1f ms (%.2f GB/s), "
+        "station->baseline %.1f us (%.0f GB/s)" % (
+            1e3 * t_write, w_gbps, 1e3 * t_read, r_gbps, 1e6 * t_k, k_gbps))
+    return {"workload": "512 stations, 10 times x 8 channels, FP32 2x2 "
+                        "cross- and auto-correlations, station uvw",
+            "block_bytes": block_bytes, "blocks": args.vis_blocks,
+            "write_ms": round(1e3 * t_write, 2),
+            "write_GBps": round(w_gbps, 2),
+            "read_ms": round(1e3 * t_read, 2),
+            "read_GBps": round(r_gbps, 2),
+            "read_back_identical": same,
+            "reference_host_write_GBps": ref_gbps,
+            "station_to_baseline_us": round(1e6 * t_k, 2),
+            "station_to_baseline_GBps": round(k_gbps, 1),
+            "station_to_baseline_frac": round(k_gbps / PEAK_HBM_GBPS, 4)}
 
 
 def bench_batch(args, st, station, work, mode):

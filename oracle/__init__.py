@@ -243,3 +243,67 @@ def factorised_subset(l, m, n, I_, E, u, v, w, wavenumber, p, q):
         u.size, _p(arr[5]), _p(arr[6]), _p(arr[7]), float(wavenumber),
         pi.size, _p(pi), _p(qi), _p(out))
     return out
+
+
+def station_to_baseline(u, v, w, num_stations, casa=False, offset_in=0,
+                        offset_out=0, out=None):
+    """Baseline (u, v, w) from station (u, v, w): the pair loop of
+    oskar/convert/src/oskar_convert_station_uvw_to_baseline_uvw.c:18-36
+    (s1 ascending, s2 from s1 + 1, b counting up from offset_out, value
+    factor (x[s2] - x[s1]) with factor -1 under the CASA convention).
+    Pure-Python loops: small cases only.  Returns (uu, vv, ww) in the input
+    dtype (arithmetic in that dtype, as the reference's FP template)."""
+    dt = np.asarray(u).dtype
+    nb = num_stations * (num_stations - 1) // 2
+    if out is None:
+        out = [np.zeros(offset_out + nb, dtype=dt) for _ in range(3)]
+    factor = dt.type(-1 if casa else 1)
+    b = 0
+    for s1 in range(num_stations):
+        i1 = s1 + offset_in
+        for s2 in range(s1 + 1, num_stations):
+            i2 = s2 + offset_in
+            for src, dst in zip((u, v, w), out):
+                dst[b + offset_out] = factor * (src[i2] - src[i1])
+            b += 1
+    return tuple(out)
+
+
+REF_BINARY_PATH = os.path.join(_HERE, "_ref", "liboskar_binary_ref.so")
+_ref_binary = None
+
+
+def ref_binary():
+    """The reference's binary container library (oskar/binary/src/*.c,
+    built by ``make -C oracle ref``) with prototypes set, or None."""
+    global _ref_binary
+    if _ref_binary is None and os.path.exists(REF_BINARY_PATH):
+        L = C.CDLL(REF_BINARY_PATH)
+        UB, SZ, IP, CP = C.c_ubyte, C.c_size_t, C.POINTER(C.c_int), C.c_char_p
+        protos = {
+            "oskar_binary_create": (P, [CP, C.c_char, IP]),
+            "oskar_binary_free": (None, [P]),
+            "oskar_binary_write": (None, [P, UB, UB, UB, I, SZ, P, IP]),
+            "oskar_binary_write_ext": (None, [P, UB, CP, CP, I, SZ, P, IP]),
+            "oskar_binary_write_int": (None, [P, UB, UB, I, I, IP]),
+            "oskar_binary_write_double": (None, [P, UB, UB, I, D, IP]),
+            "oskar_binary_num_tags": (I, [P]),
+            "oskar_binary_query": (I, [P, UB, UB, UB, I,
+                                       C.POINTER(C.c_size_t), IP]),
+            "oskar_binary_query_ext": (I, [P, UB, CP, CP, I,
+                                           C.POINTER(C.c_size_t), IP]),
+            "oskar_binary_read": (None, [P, UB, UB, UB, I, SZ, P, IP]),
+            "oskar_binary_read_ext": (None, [P, UB, CP, CP, I, SZ, P, IP]),
+            "oskar_binary_read_int": (None, [P, UB, UB, I, IP, IP]),
+            "oskar_binary_read_double": (None, [P, UB, UB, I,
+                                                C.POINTER(C.c_double), IP]),
+            "oskar_crc_create": (P, [I]),
+            "oskar_crc_free": (None, [P]),
+            "oskar_crc_compute": (C.c_ulong, [P, P, SZ]),
+        }
+        for name, (res, args) in protos.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _ref_binary = L
+    return _ref_binary

@@ -587,3 +587,54 @@ extern "C" int oskar_amd_enu_convert_launch(int precision, int kind, int num,
                 (float*) y, (float*) z);
     return (int) hipGetLastError();
 }
+
+namespace {
+
+/* Station (u, v, w) to baseline (u, v, w)
+ * (oskar/convert/define_convert_station_uvw_to_baseline_uvw.h): workgroup
+ * (p, t) takes the baselines (p, q > p) of time t, uu[b] = sign (u[q] -
+ * u[p]), sign -1 for the CASA convention.  Time t reads stations from
+ * offset_in + t num and writes baselines from offset_out + t num (num-1)/2. */
+template <typename FP>
+__global__ __launch_bounds__(THREADS)
+void station_to_baseline_kernel(int casa, int num, int offset_in,
+        const FP* __restrict__ u, const FP* __restrict__ v,
+        const FP* __restrict__ w, int offset_out, FP* __restrict__ uu,
+        FP* __restrict__ vv, FP* __restrict__ ww)
+{
+    const int p = (int) blockIdx.x, t = (int) blockIdx.y;
+    const FP sign = casa ? (FP) -1 : (FP) 1;
+    const long long in0 = offset_in + (long long) t * num;
+    const FP up = u[in0 + p], vp = v[in0 + p], wp = w[in0 + p];
+    const long long b0 = offset_out + (long long) t * num * (num - 1) / 2 +
+            (long long) p * (num - 1) - (long long) (p - 1) * p / 2 - p - 1;
+    for (int q = p + 1 + (int) threadIdx.x; q < num; q += THREADS)
+    {
+        uu[b0 + q] = sign * (u[in0 + q] - up);
+        vv[b0 + q] = sign * (v[in0 + q] - vp);
+        ww[b0 + q] = sign * (w[in0 + q] - wp);
+    }
+}
+
+} /* namespace */
+
+extern "C" int oskar_amd_station_to_baseline_launch(int precision, int casa,
+        int num_times, int num_stations, int offset_in, const void* u,
+        const void* v, const void* w, int offset_out, void* uu, void* vv,
+        void* ww, void* stream)
+{
+    if (num_stations < 2 || num_times < 1) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+    const dim3 grid(num_stations - 1, num_times);
+    if (precision == OSKAR_DOUBLE)
+        hipLaunchKernelGGL(station_to_baseline_kernel<double>, grid,
+                dim3(THREADS), 0, s, casa, num_stations, offset_in,
+                (const double*) u, (const double*) v, (const double*) w,
+                offset_out, (double*) uu, (double*) vv, (double*) ww);
+    else
+        hipLaunchKernelGGL(station_to_baseline_kernel<float>, grid,
+                dim3(THREADS), 0, s, casa, num_stations, offset_in,
+                (const float*) u, (const float*) v, (const float*) w,
+                offset_out, (float*) uu, (float*) vv, (float*) ww);
+    return (int) hipGetLastError();
+}

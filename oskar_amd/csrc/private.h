@@ -12,6 +12,7 @@
 #include "oskar_telescope.h"
 #include "oskar_station.h"
 #include "oskar_sky.h"
+#include "oskar_vis.h"
 #include "oskar_timer.h"
 
 #ifdef __cplusplus
@@ -129,6 +130,46 @@ struct oskar_Sky
     int* column_attr;
 };
 
+/* oskar/vis/private_vis_block.h */
+struct oskar_VisBlock
+{
+    int dim_start_size[6];   /* start time, start channel, times, channels,
+                                baselines, stations */
+    int has_cross_correlations;
+    int has_auto_correlations;
+    oskar_Mem* cross_correlations;
+    oskar_Mem* auto_correlations;
+    oskar_Mem* baseline_uvw_metres[3];
+    oskar_Mem* station_uvw_metres[3];
+};
+
+/* The fields of oskar/vis/private_vis_header.h that the .vis format holds. */
+struct oskar_VisHeader
+{
+    oskar_Mem* settings;
+    oskar_Mem* telescope_path;
+    int num_tags_header;
+    int num_tags_per_block;
+    int write_autocorr, write_crosscorr;
+    int amp_type, coord_precision;
+    int max_times_per_block, num_times_total;
+    int max_channels_per_block, num_channels_total;
+    int num_stations;
+    int pol_type;
+    int casa_phase_convention;
+    int phase_centre_type;
+    double phase_centre_deg[2];
+    double freq_start_hz, freq_inc_hz, channel_bandwidth_hz;
+    double time_start_mjd_utc, time_inc_sec, time_average_sec;
+    double telescope_centre_lon_deg, telescope_centre_lat_deg;
+    double telescope_centre_alt_m;
+    oskar_Mem* station_diameter_m;
+    oskar_Mem** station_name;
+    oskar_Mem* station_offset_ecef_metres[3];
+    oskar_Mem** element_enu_metres[3];
+    oskar_Mem** element_euler_angle_rad[2][3];
+};
+
 struct oskar_Timer
 {
     int type;
@@ -144,6 +185,30 @@ void* oskar_amd_current_stream(void);
 void oskar_amd_check_launch(int* status);
 int oskar_amd_hip_status(int hip_error);
 void oskar_amd_set_last_error(int error);
+
+/* ---- piecewise chunk I/O (binary.c) --------------------------------------
+ * A chunk written in pieces: begin writes the tag (data_size must equal the
+ * sum of the pieces), each piece extends the CRC, end writes it.  A chunk
+ * read in pieces: begin seeks to the payload and returns its size, pieces
+ * are read in order, end checks the CRC. */
+void oskar_amd_binary_write_begin(oskar_Binary* h, unsigned char data_type,
+        unsigned char id_group, unsigned char id_tag, int user_index,
+        size_t data_size, unsigned int* crc, int* status);
+void oskar_amd_binary_write_piece(oskar_Binary* h, const void* data,
+        size_t n, unsigned int* crc, int* status);
+void oskar_amd_binary_write_end(oskar_Binary* h, unsigned int crc,
+        int* status);
+/* Payload bytes only: the caller extends the CRC itself. */
+void oskar_amd_binary_write_raw(oskar_Binary* h, const void* data,
+        size_t n, int* status);
+void oskar_amd_binary_read_raw(oskar_Binary* h, void* data, size_t n,
+        int* status);
+size_t oskar_amd_binary_read_begin(oskar_Binary* h, int chunk_index,
+        unsigned int* crc, int* status);
+void oskar_amd_binary_read_piece(oskar_Binary* h, void* data, size_t n,
+        unsigned int* crc, int* status);
+void oskar_amd_binary_read_end(oskar_Binary* h, int chunk_index,
+        unsigned int crc, int* status);
 
 /* ---- kernel launchers (extern "C" in the .hip files) ------------------ */
 /* All return a hipError_t value (0 on success) and launch on `stream`. */
@@ -240,6 +305,10 @@ int oskar_amd_jones_K_launch(int precision, int num_sources,
         void* jones, void* stream);
 int oskar_amd_jones_join_launch(int precision, int a_matrix, int b_matrix,
         size_t n, void* out, const void* a, const void* b, void* stream);
+int oskar_amd_station_to_baseline_launch(int precision, int casa,
+        int num_times, int num_stations, int offset_in, const void* u,
+        const void* v, const void* w, int offset_out, void* uu, void* vv,
+        void* ww, void* stream);
 int oskar_amd_enu_convert_launch(int precision, int kind, int num,
         const void* a, const void* b, const void* c, const double* params,
         void* x, void* y, void* z, void* stream);
