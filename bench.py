@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--auto-steps", type=int, default=5,
                     help="oskar_auto_correlate steps on the plain "
                          "pipeline's Jones (0 = skip)")
+    ap.add_argument("--jones-steps", type=int, default=3,
+                    help="oskar_evaluate_jones_K and oskar_jones_join steps "
+                         "at 512 x 1M (0 = skip; needs the plain leg)")
     ap.add_argument("--vis-blocks", type=int, default=3,
                     help="512-station vis blocks written and read back "
                          "(0 = skip)")
@@ -276,6 +279,7 @@ def main():
     plain_rate = None
     vis_plain = None
     auto = None
+    jones = None
     if not args.no_plain:
         E = Jones(CM, OSKAR_GPU, N, S)
         K = Jones(F32 | OSKAR_COMPLEX, OSKAR_GPU, N, S)
@@ -330,6 +334,36 @@ def main():
                     "frac": round(gbps / PEAK_HBM_GBPS, 4)}
             log("auto-correlation: %.3f ms, %.0f GB/s" % (ms_auto, gbps))
             del vis_auto
+
+        # Jones K and the join alone.  K: 8 B written per (station,
+        # source) plus the source and station inputs once; join: a 2x2
+        # FP32 matrix read and written and a complex scalar read (72 B).
+        if args.jones_steps > 0:
+            def step_K():
+                lib.oskar_evaluate_jones_K(
+                    K.ptr, S, dirs[0].ptr, dirs[1].ptr, dirs[2].ptr,
+                    uvw[0].ptr, uvw[1].ptr, uvw[2].ptr, f, flux[0].ptr,
+                    -1e30, 1e30, 0, C.byref(status))
+
+            def step_join():
+                lib.oskar_jones_join(E.ptr, K.ptr, E.ptr, C.byref(status))
+            ms_k = 1e3 * timed(step_K, args.jones_steps, 1, "jones K") / \
+                args.jones_steps
+            ms_j = 1e3 * timed(step_join, args.jones_steps, 1, "join") / \
+                args.jones_steps
+            k_bytes = 8 * N * S + 16 * S + 12 * N
+            j_bytes = 72 * N * S
+            jones = {"workload": "%d stations x %d sources, FP32" % (N, S),
+                     "K_ms": round(ms_k, 4),
+                     "K_GBps": round(k_bytes / ms_k / 1e6, 1),
+                     "K_frac": round(k_bytes / ms_k / 1e6 / PEAK_HBM_GBPS,
+                                     4),
+                     "join_ms": round(ms_j, 4),
+                     "join_GBps": round(j_bytes / ms_j / 1e6, 1),
+                     "join_frac": round(j_bytes / ms_j / 1e6 /
+                                        PEAK_HBM_GBPS, 4)}
+            log("jones K %.3f ms (%.0f GB/s), join %.3f ms (%.0f GB/s)" % (
+                ms_k, k_bytes / ms_k / 1e6, ms_j, j_bytes / ms_j / 1e6))
         del E, K
 
     # --- error against FP64 ------------------------------------------
@@ -471,6 +505,7 @@ def main():
         "batch": batch,
         "surrogate": surrogate,
         "auto_correlate": auto,
+        "jones_K_join": jones,
         "horizon_clip": clip,
         "vis_block": vis,
     }

@@ -191,9 +191,14 @@ __global__ void set_value_kernel(size_t offset, size_t n, FP value, FP* a)
     }
 }
 
+/* Complex product with every product and sum rounded on its own (no fused
+ * multiply-add), so the join's result does not depend on how the compiler
+ * schedules a kernel: both join kernels below give the same bits, which are
+ * those of plain IEEE arithmetic in this term order. */
 template <typename FP>
-__device__ __forceinline__ Cplx<FP> cmul(Cplx<FP> a, Cplx<FP> b)
+__device__ __forceinline__ Cplx<FP> cmul_rn(Cplx<FP> a, Cplx<FP> b)
 {
+#pragma clang fp contract(off)
     Cplx<FP> c;
     c.x = a.x * b.x - a.y * b.y;
     c.y = a.x * b.y + a.y * b.x;
@@ -209,14 +214,14 @@ __global__ void join_kernel(size_t n, Cplx<FP>* out, const Cplx<FP>* a,
     {
         if (!AM && !BM)
         {
-            out[i] = cmul(a[i], b[i]);
+            out[i] = cmul_rn(a[i], b[i]);
         }
         else if (!AM)
         {
             const Cplx<FP> s = a[i];
             Cplx<FP> r[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) r[k] = cmul(s, b[4 * i + k]);
+            for (int k = 0; k < 4; ++k) r[k] = cmul_rn(s, b[4 * i + k]);
 #pragma unroll
             for (int k = 0; k < 4; ++k) out[4 * i + k] = r[k];
         }
@@ -225,7 +230,7 @@ __global__ void join_kernel(size_t n, Cplx<FP>* out, const Cplx<FP>* a,
             const Cplx<FP> s = b[i];
             Cplx<FP> r[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) r[k] = cmul(a[4 * i + k], s);
+            for (int k = 0; k < 4; ++k) r[k] = cmul_rn(a[4 * i + k], s);
 #pragma unroll
             for (int k = 0; k < 4; ++k) out[4 * i + k] = r[k];
         }
@@ -235,17 +240,65 @@ __global__ void join_kernel(size_t n, Cplx<FP>* out, const Cplx<FP>* a,
             const Cplx<FP>* B = b + 4 * i;
             Cplx<FP> r[4];
             /* [[a b],[c d]] x [[e f],[g h]] */
-            Cplx<FP> t0 = cmul(A[0], B[0]), t1 = cmul(A[1], B[2]);
+            Cplx<FP> t0 = cmul_rn(A[0], B[0]), t1 = cmul_rn(A[1], B[2]);
             r[0].x = t0.x + t1.x; r[0].y = t0.y + t1.y;
-            t0 = cmul(A[0], B[1]); t1 = cmul(A[1], B[3]);
+            t0 = cmul_rn(A[0], B[1]); t1 = cmul_rn(A[1], B[3]);
             r[1].x = t0.x + t1.x; r[1].y = t0.y + t1.y;
-            t0 = cmul(A[2], B[0]); t1 = cmul(A[3], B[2]);
+            t0 = cmul_rn(A[2], B[0]); t1 = cmul_rn(A[3], B[2]);
             r[2].x = t0.x + t1.x; r[2].y = t0.y + t1.y;
-            t0 = cmul(A[2], B[1]); t1 = cmul(A[3], B[3]);
+            t0 = cmul_rn(A[2], B[1]); t1 = cmul_rn(A[3], B[3]);
             r[3].x = t0.x + t1.x; r[3].y = t0.y + t1.y;
 #pragma unroll
             for (int k = 0; k < 4; ++k) out[4 * i + k] = r[k];
         }
+    }
+}
+
+/* The join on whole 2x2 matrices loaded and stored as aligned blocks
+ * (two 16-byte accesses per FP32 matrix, four per FP64), one element per
+ * thread and iteration; arithmetic as join_kernel, term for term.  Used
+ * when every pointer is aligned to a matrix. */
+template <typename FP> struct alignas(4 * sizeof(Cplx<FP>)) JonesMat
+{
+    Cplx<FP> v[4];
+};
+
+template <typename FP, bool AM, bool BM>
+__global__ __launch_bounds__(THREADS)
+void join_vec_kernel(size_t n, JonesMat<FP>* out, const void* a, const void* b)
+{
+    GRID_STRIDE(i, n)
+    {
+        JonesMat<FP> r;
+        if (AM && BM)
+        {
+            const JonesMat<FP> A = ((const JonesMat<FP>*) a)[i];
+            const JonesMat<FP> B = ((const JonesMat<FP>*) b)[i];
+            Cplx<FP> t0 = cmul_rn(A.v[0], B.v[0]);
+            Cplx<FP> t1 = cmul_rn(A.v[1], B.v[2]);
+            r.v[0].x = t0.x + t1.x; r.v[0].y = t0.y + t1.y;
+            t0 = cmul_rn(A.v[0], B.v[1]); t1 = cmul_rn(A.v[1], B.v[3]);
+            r.v[1].x = t0.x + t1.x; r.v[1].y = t0.y + t1.y;
+            t0 = cmul_rn(A.v[2], B.v[0]); t1 = cmul_rn(A.v[3], B.v[2]);
+            r.v[2].x = t0.x + t1.x; r.v[2].y = t0.y + t1.y;
+            t0 = cmul_rn(A.v[2], B.v[1]); t1 = cmul_rn(A.v[3], B.v[3]);
+            r.v[3].x = t0.x + t1.x; r.v[3].y = t0.y + t1.y;
+        }
+        else if (AM)
+        {
+            const JonesMat<FP> A = ((const JonesMat<FP>*) a)[i];
+            const Cplx<FP> sc = ((const Cplx<FP>*) b)[i];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r.v[k] = cmul_rn(A.v[k], sc);
+        }
+        else
+        {
+            const Cplx<FP> sc = ((const Cplx<FP>*) a)[i];
+            const JonesMat<FP> B = ((const JonesMat<FP>*) b)[i];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r.v[k] = cmul_rn(sc, B.v[k]);
+        }
+        out[i] = r;
     }
 }
 
@@ -435,9 +488,18 @@ extern "C" int oskar_amd_jones_join_launch(int precision, int a_matrix,
 {
     if (n == 0) return 0;
     const hipStream_t s = (hipStream_t) stream;
-#define JOIN(FP, AM, BM) hipLaunchKernelGGL((join_kernel<FP, AM, BM>),       \
-        grid_for(n), dim3(THREADS), 0, s, n, (Cplx<FP>*) out,                \
-        (const Cplx<FP>*) a, (const Cplx<FP>*) b)
+    const size_t mat = (precision == OSKAR_DOUBLE ? 16 : 8) * 4;
+    const bool aligned = ((size_t) out % mat == 0) &&
+            (!a_matrix || (size_t) a % mat == 0) &&
+            (!b_matrix || (size_t) b % mat == 0);
+#define JOIN(FP, AM, BM)                                                     \
+    if ((AM || BM) && aligned)                                               \
+        hipLaunchKernelGGL((join_vec_kernel<FP, AM, BM>), grid_for(n),       \
+                dim3(THREADS), 0, s, n, (JonesMat<FP>*) out, a, b);              \
+    else                                                                     \
+        hipLaunchKernelGGL((join_kernel<FP, AM, BM>), grid_for(n),           \
+                dim3(THREADS), 0, s, n, (Cplx<FP>*) out,                     \
+                (const Cplx<FP>*) a, (const Cplx<FP>*) b)
     if (precision == OSKAR_DOUBLE)
     {
         if (a_matrix && b_matrix) JOIN(double, true, true);
@@ -458,7 +520,13 @@ extern "C" int oskar_amd_jones_join_launch(int precision, int a_matrix,
 
 namespace {
 
-/* K[a][s] (define_evaluate_jones_K.h:56-75, CPU evaluation order). */
+/* K[a][s] (define_evaluate_jones_K.h:56-75, CPU evaluation order).  A
+ * thread holds one source's direction and filter value and walks a group of
+ * K_GROUP stations (wave-uniform station values), so the source inputs are
+ * read once per group and each store of a wave covers 64 consecutive
+ * sources of one station row. */
+constexpr int K_GROUP = 16;
+
 template <typename FP>
 __global__ __launch_bounds__(256)
 void jones_K_kernel(int num_sources, const FP* l, const FP* m, const FP* n,
@@ -467,19 +535,25 @@ void jones_K_kernel(int num_sources, const FP* l, const FP* m, const FP* n,
         Cplx<FP>* jones)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = blockIdx.y;
-    if (s >= num_sources || a >= num_stations) return;
-    Cplx<FP> k;
-    k.x = k.y = (FP) 0;
+    const int a0 = blockIdx.y * K_GROUP;
+    if (s >= num_sources) return;
     const FP f = filter[s];
-    if (f > fmin && f <= fmax)
+    const bool keep = f > fmin && f <= fmax;
+    const FP ls = l[s], ms = m[s], ns1 = n[s] - (FP) 1;
+    const int a1 = min(a0 + K_GROUP, num_stations);
+    for (int a = a0; a < a1; ++a)
     {
-        FP phase = u[a] * l[s] + v[a] * m[s];
-        if (!ignore_w) phase += w[a] * (n[s] - (FP) 1);
-        phase *= wavenumber;
-        sincos_v<FP>(phase, &k.y, &k.x);
+        Cplx<FP> k;
+        k.x = k.y = (FP) 0;
+        if (keep)
+        {
+            FP phase = u[a] * ls + v[a] * ms;
+            if (!ignore_w) phase += w[a] * ns1;
+            phase *= wavenumber;
+            sincos_v<FP>(phase, &k.y, &k.x);
+        }
+        jones[(size_t) s + (size_t) num_sources * a] = k;
     }
-    jones[(size_t) s + (size_t) num_sources * a] = k;
 }
 
 } /* namespace */
@@ -491,7 +565,8 @@ extern "C" int oskar_amd_jones_K_launch(int precision, int num_sources,
         void* jones, void* stream)
 {
     if (num_sources <= 0 || num_stations <= 0) return 0;
-    const dim3 grid((num_sources + 255) / 256, num_stations);
+    const dim3 grid((num_sources + 255) / 256,
+            (num_stations + K_GROUP - 1) / K_GROUP);
     const hipStream_t s = (hipStream_t) stream;
     if (precision == OSKAR_DOUBLE)
         hipLaunchKernelGGL(jones_K_kernel<double>, grid, dim3(256), 0, s,
