@@ -185,6 +185,42 @@ void oskar_sky_horizon_clip(oskar_Sky* out, const oskar_Sky* in,
         const oskar_Telescope* telescope, double gast_rad,
         oskar_StationWork* work, int* status);
 
+/* oskar/sky/src/oskar_sky_evaluate_relative_directions.c:20-37: the
+ * scratch columns L, M, N (created if missing) get the direction cosines
+ * of each source about (ra0, dec0):
+ *   l = cos(dec) sin(ra - ra0),
+ *   m = cos(dec0) sin(dec) - sin(dec0) cos(dec) cos(ra - ra0),
+ *   n = sin(dec0) sin(dec) + cos(dec0) cos(dec) cos(ra - ra0),
+ * and the sky's reference point is set to (ra0, dec0).  A sky in host
+ * memory is staged through device memory; the arithmetic runs on the GPU
+ * either way.  OSKAR_ERR_INVALID_ARGUMENT if the RA or Dec column is
+ * missing. */
+OSKAR_EXPORT
+void oskar_sky_evaluate_relative_directions(oskar_Sky* sky, double ra0_rad,
+        double dec0_rad, int* status);
+
+/* oskar/sky/src/oskar_sky_evaluate_gaussian_source_parameters.c:32-262:
+ * for each source with a non-zero major or minor FWHM, the scratch columns
+ * EXT_A, EXT_B, EXT_C (created if missing) get the Gaussian parameters
+ * a, b, c that the correlator's exp(-(a u^2 + 2 b u v + c v^2)) uses, from
+ * an ellipse fitted to six points of the source's ellipse projected onto
+ * the tangent plane at (ra0, dec0).  Sources whose fit fails keep their
+ * a, b, c and add one to *num_failed (which is not cleared first).
+ * zero_failed_sources changes nothing, as in the reference, whose
+ * flux-clearing call returns at once because *status still holds the fit's
+ * error code at that point (:152-160); failed sources keep their fluxes.
+ * Returns without doing anything if the major, minor or position-angle
+ * column is missing.
+ * The reference accepts only a sky in host memory and fits serially on the
+ * CPU; here each source is fitted by one GPU thread.  A sky in device
+ * memory is processed in place; one in host memory is staged through device
+ * memory.  The call waits for the kernel (it returns the failure count).
+ * OSKAR_ERR_INVALID_ARGUMENT if the RA or Dec column is missing. */
+OSKAR_EXPORT
+void oskar_sky_evaluate_gaussian_source_parameters(oskar_Sky* sky,
+        int zero_failed_sources, double ra0, double dec0, int* num_failed,
+        int* status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,14 @@ def lib():
         ac = [I, I, I, I] + [P] * 6
         _lib.oracle_auto_correlate_d.argtypes = ac
         _lib.oracle_auto_correlate_f.argtypes = ac
+        rd = [I, P, P, D, D, P, P, P]
+        _lib.oracle_relative_directions_d.argtypes = rd
+        _lib.oracle_relative_directions_f.argtypes = rd
+        gp = [I] + [P] * 5 + [D, D] + [P] * 4
+        _lib.oracle_gaussian_parameters_d.argtypes = gp
+        _lib.oracle_gaussian_parameters_f.argtypes = gp
+        _lib.oracle_gaussian_parameters_d.restype = I
+        _lib.oracle_gaussian_parameters_f.restype = I
     return _lib
 
 
@@ -307,3 +315,63 @@ def ref_binary():
             fn.argtypes = args
         _ref_binary = L
     return _ref_binary
+
+
+def relative_directions(precision, ra, dec, ra0, dec0):
+    """(l, m, n) of (ra, dec) about (ra0, dec0) in the precision given."""
+    dt = np.float64 if precision == "d" else np.float32
+    ra, dec = _c(ra, dt), _c(dec, dt)
+    n = ra.size
+    l, m, nn = (np.zeros(n, dt) for _ in range(3))
+    getattr(lib(), "oracle_relative_directions_" + precision)(
+        n, _p(ra), _p(dec), ra0, dec0, _p(l), _p(m), _p(nn))
+    return l, m, nn
+
+
+def gaussian_parameters(precision, ra, dec, maj, min_, pa, ra0, dec0,
+                        a=None, b=None, c=None):
+    """Gaussian a, b, c and the per-source fit-failure flags; a, b, c start
+    from the given arrays (zeros if None) and keep their values where a
+    source is skipped or fails.  Returns (a, b, c, failed)."""
+    dt = np.float64 if precision == "d" else np.float32
+    ra, dec, maj, min_, pa = (_c(x, dt) for x in (ra, dec, maj, min_, pa))
+    n = ra.size
+    a, b, c = (np.zeros(n, dt) if x is None else np.array(x, dt)
+               for x in (a, b, c))
+    failed = np.zeros(n, np.int32)
+    getattr(lib(), "oracle_gaussian_parameters_" + precision)(
+        n, _p(ra), _p(dec), _p(maj), _p(min_), _p(pa), ra0, dec0, _p(a),
+        _p(b), _p(c), _p(failed))
+    return a, b, c, failed
+
+
+REF_SKY_PATH = os.path.join(_HERE, "_ref", "liboskar_sky_ref.so")
+_ref_sky = None
+
+
+def ref_sky():
+    """The reference's Gaussian source parameter function with the sky
+    table, oskar_Mem core, ellipse fit and conversions it calls, compiled
+    from its sources by ``make -C oracle ref``; None if not built."""
+    global _ref_sky
+    if _ref_sky is None and os.path.exists(REF_SKY_PATH):
+        L = C.CDLL(REF_SKY_PATH)
+        IP = C.POINTER(C.c_int)
+        protos = {
+            "oskar_sky_create": (P, [I, I, I, IP]),
+            "oskar_sky_free": (None, [P, IP]),
+            "oskar_sky_column": (P, [P, I, I, IP]),
+            "oskar_mem_void": (P, [P]),
+            "oskar_sky_evaluate_gaussian_source_parameters": (
+                None, [P, I, D, D, IP, IP]),
+            "ref_lon_lat_to_relative_directions_3d_float": (
+                None, [I, P, P, C.c_float, C.c_float, C.c_float, P, P, P]),
+            "ref_lon_lat_to_relative_directions_3d_double": (
+                None, [I, P, P, D, D, D, P, P, P]),
+        }
+        for name, (res, args) in protos.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _ref_sky = L
+    return _ref_sky

@@ -739,3 +739,256 @@ void oracle_factorised_subset_ld(int num_sources, const double* l,
         for (r = 0; r < 8; ++r) vis_out[8 * (size_t) k + r] = (double) acc[r];
     }
 }
+
+/*
+ * Relative direction cosines of sky positions (the 3D CPU body of
+ * oskar/convert/define_convert_lon_lat_to_relative_directions.h:3-17, as
+ * called by oskar/sky/src/oskar_sky_evaluate_relative_directions.c:20-37
+ * with lon0 = ra0, cos_lat0 = cos(dec0), sin_lat0 = sin(dec0)):
+ *   l = cos(lat) sin(lon - lon0)
+ *   m = cos(lat0) sin(lat) - sin(lat0) cos(lat) cos(lon - lon0)
+ *   n = sin(lat0) sin(lat) + cos(lat0) cos(lat) cos(lon - lon0)
+ * The reference compiles these macro bodies as C99 with <tgmath.h>
+ * (oskar/utility/oskar_kernel_macros.h:207-209), so its float instance calls
+ * sinf and cosf: TSIN and TCOS name them here.
+ */
+#define REL_DIR_BODY(NAME, FP, TSIN, TCOS)                                    \
+void NAME(int num, const FP* lon_rad, const FP* lat_rad, double lon0,         \
+        double lat0, FP* l, FP* m, FP* n)                                     \
+{                                                                             \
+    int i;                                                                    \
+    const FP lon0_ = (FP) lon0, cos_lat0 = (FP) cos(lat0),                    \
+            sin_lat0 = (FP) sin(lat0);                                        \
+    for (i = 0; i < num; ++i)                                                 \
+    {                                                                         \
+        FP sin_lon, cos_lon, sin_lat, cos_lat;                                \
+        const FP lon = lon_rad[i] - lon0_, lat = lat_rad[i];                  \
+        sin_lon = TSIN(lon); cos_lon = TCOS(lon);                             \
+        sin_lat = TSIN(lat); cos_lat = TCOS(lat);                             \
+        l[i] = cos_lat * sin_lon;                                             \
+        m[i] = cos_lat0 * sin_lat - sin_lat0 * cos_lat * cos_lon;             \
+        n[i] = sin_lat0 * sin_lat + cos_lat0 * cos_lat * cos_lon;             \
+    }                                                                         \
+}
+REL_DIR_BODY(oracle_relative_directions_d, double, sin, cos)
+REL_DIR_BODY(oracle_relative_directions_f, float, sinf, cosf)
+
+/*
+ * Gaussian source parameters (oskar/sky/src/
+ * oskar_sky_evaluate_gaussian_source_parameters.c:32-262).  Per source with
+ * major or minor FWHM non-zero:
+ *  1. six points of the ellipse (FWHM/2 semi-axes, position angle PA) on
+ *     the tangent plane at (0, 0), t_j = 60 j degrees (:99-140);
+ *  2. tangent plane -> lon/lat at (0, 0) -> xyz, rotated on the sphere to
+ *     the source's (RA, Dec), -> lon/lat, -> tangent plane at the phase
+ *     centre (:141-149; oskar/math/src/oskar_rotate.c:31-48,
+ *     oskar/convert/src/oskar_convert_lon_lat_to_xyz.c,
+ *     oskar_convert_xyz_to_lon_lat.c, and the two 2D macro bodies);
+ *  3. least-squares conic a x^2 + b xy + c y^2 + d x + e y = 1 through the
+ *     mean-removed points by the normal equations (X^T X) p = X^T 1, solved
+ *     by LU with partial pivoting (oskar/math/src/oskar_fit_ellipse.c:17-
+ *     131; oskar_matrix_multiply.c with transpose_a; oskar_lapack_subset.c
+ *     getrf2/getrs).  The recursive getrf2 applies the same rank-1 updates
+ *     in the same column order as this right-looking loop (the reference's
+ *     gemm adds (-u) l, the same rounding as subtracting u l), so the
+ *     factors round identically.  A zero pivot, or a*c <= 0 after the
+ *     rotation that removes b, marks the fit as failed;
+ *  4. a, b, c of the Gaussian in the uv plane from the fitted FWHMs and PA
+ *     (:170-178 and :249-257).
+ * NSIN/NCOS/NATAN2/NSQRT/NASIN are the libm functions of the precision:
+ * the reference's per-precision helpers call sinf etc. in their float
+ * versions, and its two 2D conversion macro bodies are compiled with
+ * <tgmath.h>, which picks the same float functions for float arguments.
+ * The ellipse fit and the loop body of the sky function use the double
+ * functions in both precisions, as the reference's plain C does.  A failed
+ * source gets failed[i] = 1 and keeps its a, b, c; a source with both axes
+ * zero is skipped (failed[i] = 0, a, b, c untouched).
+ */
+#define GAUSS_PARAMS_BODY(NAME, FP, FP_MIN, NSIN, NCOS, NATAN2, NSQRT,  \
+        NASIN)                                                                \
+static int NAME##_fit(FP* major, FP* minor, FP* pa_out, const FP* x,          \
+        const FP* y)                                                          \
+{                                                                             \
+    int i, j, k, ok = 1;                                                      \
+    FP mean_x = 0, mean_y = 0, X[6][5], XX[5][5], s[5];                       \
+    FP a, b, c, d, e;                                                         \
+    for (i = 0; i < 5; ++i) s[i] = 0;                                         \
+    for (i = 0; i < 6; ++i) { mean_x += x[i]; mean_y += y[i]; }               \
+    mean_x /= (FP) 6;                                                         \
+    mean_y /= (FP) 6;                                                         \
+    for (i = 0; i < 6; ++i)                                                   \
+    {                                                                         \
+        const FP x_ = x[i] - mean_x, y_ = y[i] - mean_y;                      \
+        X[i][0] = x_ * x_; X[i][1] = x_ * y_; X[i][2] = y_ * y_;              \
+        X[i][3] = x_; X[i][4] = y_;                                           \
+        for (j = 0; j < 5; ++j) s[j] += X[i][j];                              \
+    }                                                                         \
+    /* XX[i][j] = sum_k X[k][j] X[k][i], skipping zero X[k][j]. */            \
+    for (j = 0; j < 5; ++j)                                                   \
+        for (i = 0; i < 5; ++i)                                               \
+        {                                                                     \
+            FP acc = 0;                                                       \
+            for (k = 0; k < 6; ++k)                                           \
+                if (X[k][j] != 0) acc += X[k][j] * X[k][i];                   \
+            XX[i][j] = acc;                                                   \
+        }                                                                     \
+    /* LU with partial pivoting (first largest magnitude wins). */            \
+    {                                                                         \
+        int piv[5];                                                           \
+        for (k = 0; k < 5; ++k)                                               \
+        {                                                                     \
+            int p = k;                                                        \
+            FP mx = fabs(XX[k][k]);                                           \
+            for (i = k + 1; i < 5; ++i)                                       \
+                if (fabs(XX[i][k]) > mx) { mx = fabs(XX[i][k]); p = i; }      \
+            piv[k] = p;                                                       \
+            if (XX[p][k] != 0)                                                \
+            {                                                                 \
+                if (p != k)                                                   \
+                    for (j = 0; j < 5; ++j)                                   \
+                    { FP t = XX[k][j]; XX[k][j] = XX[p][j]; XX[p][j] = t; }   \
+                if (fabs(XX[k][k]) >= FP_MIN)                                 \
+                {                                                             \
+                    const FP r = (FP) 1 / XX[k][k];                           \
+                    for (i = k + 1; i < 5; ++i) XX[i][k] = r * XX[i][k];      \
+                }                                                             \
+                else                                                          \
+                    for (i = k + 1; i < 5; ++i) XX[i][k] /= XX[k][k];         \
+            }                                                                 \
+            else ok = 0;                                                      \
+            for (j = k + 1; j < 5; ++j)                                       \
+                if (XX[k][j] != 0)                                            \
+                    for (i = k + 1; i < 5; ++i)                               \
+                        XX[i][j] -= XX[k][j] * XX[i][k];                      \
+        }                                                                     \
+        for (k = 0; k < 5; ++k)                                               \
+            if (piv[k] != k)                                                  \
+            { FP t = s[k]; s[k] = s[piv[k]]; s[piv[k]] = t; }                 \
+        for (k = 0; k < 5; ++k)                                               \
+            if (s[k] != 0)                                                    \
+                for (i = k + 1; i < 5; ++i) s[i] -= s[k] * XX[i][k];          \
+        for (k = 4; k >= 0; --k)                                              \
+            if (s[k] != 0)                                                    \
+            {                                                                 \
+                s[k] /= XX[k][k];                                             \
+                for (i = 0; i < k; ++i) s[i] -= s[k] * XX[i][k];              \
+            }                                                                 \
+    }                                                                         \
+    a = s[0]; b = s[1]; c = s[2]; d = s[3]; e = s[4];                         \
+    {                                                                         \
+        const double r1 = fabs(b / a), r2 = fabs(b / c);                      \
+        if ((r1 < r2 ? r1 : r2) > 1e-3)                                       \
+        {                                                                     \
+            FP a2, c2, d2, e2;                                                \
+            *pa_out = 0.5 * atan2(b, (c - a)) - M_PI / 2.0;                   \
+            const FP cos_phi = (FP) cos(*pa_out);                             \
+            const FP sin_phi = (FP) sin(*pa_out);                             \
+            a2 = a*cos_phi*cos_phi - b*cos_phi*sin_phi + c*sin_phi*sin_phi;   \
+            c2 = a*sin_phi*sin_phi + b*cos_phi*sin_phi + c*cos_phi*cos_phi;   \
+            d2 = d*cos_phi - e*sin_phi;                                       \
+            e2 = d*sin_phi + e*cos_phi;                                       \
+            a = a2; c = c2; d = d2; e = e2;                                   \
+        }                                                                     \
+        else *pa_out = (c - a < 0.0) ? 0.0 : M_PI / 2.0;                      \
+    }                                                                         \
+    if (a * c <= 0.0) return 0;                                               \
+    if (a < 0.0) { a = -a; c = -c; d = -d; e = -e; }                          \
+    {                                                                         \
+        const FP F = 1.0 + (d * d) / (4.0 * a) + (e * e) / (4.0 * c);         \
+        a = sqrt(F / a);                                                      \
+        b = sqrt(F / c);                                                      \
+        *major = 2.0 * (a > b ? a : b);                                       \
+        *minor = 2.0 * (a < b ? a : b);                                       \
+    }                                                                         \
+    return ok;                                                                \
+}                                                                             \
+int NAME(int num, const FP* ra, const FP* dec, const FP* maj_, const FP* min_,\
+        const FP* pa_, double ra0, double dec0, FP* a_, FP* b_, FP* c_,       \
+        int* failed)                                                          \
+{                                                                             \
+    int i, j, num_failed = 0;                                                 \
+    const double sin_dec0 = sin(dec0), cos_dec0 = cos(dec0);                  \
+    const FP lon0 = (FP) ra0, cl0 = (FP) cos_dec0, sl0 = (FP) sin_dec0;       \
+    for (i = 0; i < num; ++i)                                                 \
+    {                                                                         \
+        FP l[6], m[6], lon[6], lat[6], x[6], y[6], z[6];                      \
+        FP maj, min, pa;                                                      \
+        failed[i] = 0;                                                        \
+        if (maj_[i] == 0.0 && min_[i] == 0.0) continue;                       \
+        const FP ea = maj_[i] / 2.0, eb = min_[i] / 2.0;                      \
+        const FP cos_pa = cos(pa_[i]), sin_pa = sin(pa_[i]);                  \
+        for (j = 0; j < 6; ++j)                                               \
+        {                                                                     \
+            const FP t = j * 60.0 * M_PI / 180.0;                             \
+            l[j] = ea * cos(t) * sin_pa + eb * sin(t) * cos_pa;               \
+            m[j] = ea * cos(t) * cos_pa - eb * sin(t) * sin_pa;               \
+        }                                                                     \
+        /* Tangent plane at (0, 0) -> lon, lat (2D macro, lon0 = 0). */       \
+        for (j = 0; j < 6; ++j)                                               \
+        {                                                                     \
+            const FP l0 = l[j], m0 = m[j];                                    \
+            const FP n0 = NSQRT((FP) 1 - l0 * l0 - m0 * m0);                  \
+            lat[j] = NASIN(n0 * (FP) 0 + m0 * (FP) 1);                        \
+            lon[j] = (FP) 0 + NATAN2(l0, (FP) 1 * n0 - m0 * (FP) 0);          \
+        }                                                                     \
+        for (j = 0; j < 6; ++j)                                               \
+        {                                                                     \
+            const FP cos_lon = NCOS(lon[j]), sin_lon = NSIN(lon[j]);          \
+            const FP cos_lat = NCOS(lat[j]), sin_lat = NSIN(lat[j]);          \
+            x[j] = cos_lat * cos_lon;                                         \
+            y[j] = cos_lat * sin_lon;                                         \
+            z[j] = sin_lat;                                                   \
+        }                                                                     \
+        {                                                                     \
+            const FP cos_lon = NCOS(ra[i]), sin_lon = NSIN(ra[i]);            \
+            const FP cos_lat = NCOS(dec[i]), sin_lat = NSIN(dec[i]);          \
+            for (j = 0; j < 6; ++j)                                           \
+            {                                                                 \
+                const FP x_ = x[j], y_ = y[j], z_ = z[j];                     \
+                x[j] = x_ * cos_lon * cos_lat - y_ * sin_lon                  \
+                        - z_ * cos_lon * sin_lat;                             \
+                y[j] = x_ * cos_lat * sin_lon + y_ * cos_lon                  \
+                        - z_ * sin_lon * sin_lat;                             \
+                z[j] = x_ * sin_lat + z_ * cos_lat;                           \
+            }                                                                 \
+        }                                                                     \
+        for (j = 0; j < 6; ++j)                                               \
+        {                                                                     \
+            const FP x_ = x[j], y_ = y[j], z_ = z[j];                         \
+            lon[j] = NATAN2(y_, x_);                                          \
+            lat[j] = NATAN2(z_, NSQRT(x_ * x_ + y_ * y_));                    \
+        }                                                                     \
+        for (j = 0; j < 6; ++j)                                               \
+        {                                                                     \
+            FP sin_lon, cos_lon, sin_lat, cos_lat;                            \
+            const FP lo = lon[j] - lon0, la = lat[j];                         \
+            sin_lon = NSIN(lo); cos_lon = NCOS(lo);                           \
+            sin_lat = NSIN(la); cos_lat = NCOS(la);                           \
+            l[j] = cos_lat * sin_lon;                                         \
+            m[j] = cl0 * sin_lat - sl0 * cos_lat * cos_lon;                   \
+        }                                                                     \
+        if (!NAME##_fit(&maj, &min, &pa, l, m))                               \
+        {                                                                     \
+            failed[i] = 1;                                                    \
+            ++num_failed;                                                     \
+            continue;                                                         \
+        }                                                                     \
+        {                                                                     \
+            const FP inv_std_maj_2 = 0.5 * (maj * maj) * M_PI_2_2_LN_2;       \
+            const FP inv_std_min_2 = 0.5 * (min * min) * M_PI_2_2_LN_2;       \
+            const FP cos_pa_2 = cos(pa) * cos(pa);                            \
+            const FP sin_pa_2 = sin(pa) * sin(pa);                            \
+            const FP sin_2pa = sin(2.0 * pa);                                 \
+            a_[i] = cos_pa_2 * inv_std_min_2 + sin_pa_2 * inv_std_maj_2;      \
+            b_[i] = -sin_2pa * inv_std_min_2 * 0.5                            \
+                    + sin_2pa * inv_std_maj_2 * 0.5;                          \
+            c_[i] = sin_pa_2 * inv_std_min_2 + cos_pa_2 * inv_std_maj_2;      \
+        }                                                                     \
+    }                                                                         \
+    return num_failed;                                                        \
+}
+#define M_PI_2_2_LN_2 7.11941466249375271693034 /* pi^2 / (2 ln 2) */
+GAUSS_PARAMS_BODY(oracle_gaussian_parameters_d, double, DBL_MIN, sin, cos,
+        atan2, sqrt, asin)
+GAUSS_PARAMS_BODY(oracle_gaussian_parameters_f, float, FLT_MIN, sinf, cosf,
+        atan2f, sqrtf, asinf)

@@ -152,6 +152,28 @@ void oracle_factorised_subset_ld(int num_sources, const double* l,
         double wavenumber, int num_baselines, const int* p, const int* q,
         double* vis_out);
 
+/* Relative direction cosines of sky positions about (lon0, lat0)
+ * (oskar/convert/define_convert_lon_lat_to_relative_directions.h, 3D body;
+ * oskar/sky/src/oskar_sky_evaluate_relative_directions.c:20-37). */
+void oracle_relative_directions_d(int num, const double* lon_rad,
+        const double* lat_rad, double lon0, double lat0, double* l,
+        double* m, double* n);
+void oracle_relative_directions_f(int num, const float* lon_rad,
+        const float* lat_rad, double lon0, double lat0, float* l, float* m,
+        float* n);
+
+/* Gaussian source parameters a, b, c from FWHMs and position angles
+ * (oskar/sky/src/oskar_sky_evaluate_gaussian_source_parameters.c:32-262):
+ * failed[i] = 1 where the ellipse fit fails (a, b, c left as they were);
+ * sources with both axes zero are skipped.  Returns the number failed. */
+int oracle_gaussian_parameters_d(int num, const double* ra,
+        const double* dec, const double* maj, const double* min,
+        const double* pa, double ra0, double dec0, double* a, double* b,
+        double* c, int* failed);
+int oracle_gaussian_parameters_f(int num, const float* ra, const float* dec,
+        const float* maj, const float* min, const float* pa, double ra0,
+        double dec0, float* a, float* b, float* c, int* failed);
+
 #ifdef __cplusplus
 }
 #endif

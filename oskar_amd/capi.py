@@ -283,6 +283,9 @@ SIGNATURES = {
     "oskar_prefix_sum": (None, [SZ, P, P, IP]),
     "oskar_sky_copy_source_data": (None, [P, P, P, P, IP]),
     "oskar_sky_horizon_clip": (None, [P, P, P, D, P, IP]),
+    "oskar_sky_evaluate_relative_directions": (None, [P, D, D, IP]),
+    "oskar_sky_evaluate_gaussian_source_parameters": (
+        None, [P, I, D, D, IP, IP]),
     "oskar_station_work_free": (None, [P, IP]),
     "oskar_station_beam_horizon_direction": (None, [P, D, DP, DP, DP, IP]),
     "oskar_evaluate_station_beam_aperture_array": (None, [P, P, I, P, P, P, I,

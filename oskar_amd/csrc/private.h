@@ -313,6 +313,23 @@ int oskar_amd_enu_convert_launch(int precision, int kind, int num,
         const void* a, const void* b, const void* c, const double* params,
         void* x, void* y, void* z, void* stream);
 
+/* sky_params.hip */
+int oskar_amd_relative_directions_launch(int precision, int num,
+        const void* lon, const void* lat, double lon0, double cos_lat0,
+        double sin_lat0, void* l, void* m, void* n, void* stream);
+/* Columns of one sky table that the Gaussian parameter kernel reads and
+ * writes, as column numbers (column k starts at k * capacity), with the
+ * cosines and sines of the six ellipse angles from the host's libm. */
+typedef struct oskar_GaussianColumns
+{
+    size_t ra, dec, maj, min, pa, a, b, c;
+    double cos_t[6], sin_t[6];
+} oskar_GaussianColumns;
+/* Clears *num_failed_dev (device memory), then adds one per failed fit. */
+int oskar_amd_gaussian_params_launch(int precision, int num, void* table,
+        size_t capacity, const oskar_GaussianColumns* cols, double ra0,
+        double cos_dec0, double sin_dec0, int* num_failed_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
