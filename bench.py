@@ -122,6 +122,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--clip-steps", type=int, default=10,
                     help="horizon clips of a 1M-source catalogue (0 = skip)")
+    ap.add_argument("--sky-steps", type=int, default=5,
+                    help="relative directions and Gaussian parameters of a "
+                         "1M-source sky (0 = skip)")
     ap.add_argument("--auto-steps", type=int, default=5,
                     help="oskar_auto_correlate steps on the plain "
                          "pipeline's Jones (0 = skip)")
@@ -414,6 +417,9 @@ def main():
     clip = None
     if args.clip_steps > 0:
         clip = bench_clip(args, rng)
+    sky_params = None
+    if args.sky_steps > 0:
+        sky_params = bench_sky_params(args, rng)
 
     # --- batched (channel, time) units in one persistent launch ---------
     batch = None
@@ -507,6 +513,7 @@ def main():
         "auto_correlate": auto,
         "jones_K_join": jones,
         "horizon_clip": clip,
+        "sky_parameters": sky_params,
         "vis_block": vis,
     }
     print(json.dumps(line), flush=True)
@@ -583,6 +590,102 @@ def bench_clip(args, rng):
             "achieved_GBps": round(gbps, 1),
             "hbm_peak_GBps": PEAK_HBM_GBPS,
             "frac": round(gbps / PEAK_HBM_GBPS, 4)}
+
+
+def bench_sky_params(args, rng):
+    """oskar_sky_evaluate_relative_directions and
+    oskar_sky_evaluate_gaussian_source_parameters on a 1M-source sky in
+    device memory (sources within 40 deg of the phase centre, 30 %
+    Gaussian with FWHM 5-60 arcsec, axis ratio 0.2-1), FP64 and FP32.  The
+    Gaussian call includes its 4-byte read of the failure count.  The
+    reference's own function (oracle/_ref, serial on the host, as the
+    reference runs it) is timed on the first 100k sources of the FP64 sky.
+    Bytes per source: relative directions 2 read + 3 written values;
+    Gaussian parameters 5 read + 3 written (the 70 % point sources read 2
+    and write nothing)."""
+    import oracle
+    from oskar_amd import lib
+    from oskar_amd.mem import (OSKAR_DOUBLE, OSKAR_GPU, OSKAR_SINGLE, Sky,
+                               check_status)
+    S = 1_000_000
+    ra0, dec0 = 0.5, -0.45
+    r = np.radians(40.0) * np.sqrt(rng.uniform(0, 1, S))
+    th = rng.uniform(0, 2 * np.pi, S)
+    l, m = np.sin(r) * np.sin(th), np.sin(r) * np.cos(th)
+    nn = np.sqrt(1 - l * l - m * m)
+    dec = np.arcsin(nn * np.sin(dec0) + m * np.cos(dec0))
+    ra = ra0 + np.arctan2(l, nn * np.cos(dec0) - m * np.sin(dec0))
+    gauss = rng.uniform(0, 1, S) < 0.3
+    maj = np.where(gauss, np.radians(rng.uniform(5, 60, S) / 3600), 0.0)
+    mn = maj * rng.uniform(0.2, 1.0, S)
+    pa = rng.uniform(0, np.pi, S)
+    cols = {1: ra, 2: dec, 10: maj, 11: mn, 12: pa}
+    out = {"workload": "1M-source sky within 40 deg of the phase centre, "
+                       "30 % Gaussian (FWHM 5-60 arcsec)"}
+    status = C.c_int(0)
+    timer = lib.oskar_timer_create(1)
+    for prec, code in (("f64", OSKAR_DOUBLE), ("f32", OSKAR_SINGLE)):
+        sky = Sky(code, OSKAR_GPU, S)
+        for t, v in cols.items():
+            sky.set_column(t, v)
+        nf = C.c_int(0)
+
+        def rel():
+            lib.oskar_sky_evaluate_relative_directions(
+                sky.ptr, ra0, dec0, C.byref(status))
+
+        def gau():
+            lib.oskar_sky_evaluate_gaussian_source_parameters(
+                sky.ptr, 0, ra0, dec0, C.byref(nf), C.byref(status))
+
+        res = {}
+        for name, fn in (("relative_directions", rel),
+                         ("gaussian_parameters", gau)):
+            fn()
+            lib.oskar_device_synchronize(OSKAR_GPU)
+            lib.oskar_timer_start(timer)
+            for _ in range(args.sky_steps):
+                fn()
+            lib.oskar_device_synchronize(OSKAR_GPU)
+            res[name] = 1e3 * lib.oskar_timer_elapsed(timer) / args.sky_steps
+        check_status(status, "sky parameters")
+        out[prec] = {"relative_directions_ms": round(
+                         res["relative_directions"], 4),
+                     "gaussian_parameters_ms": round(
+                         res["gaussian_parameters"], 4),
+                     "failed": nf.value // (args.sky_steps + 1)}
+        log("sky parameters %s: relative directions %.3f ms, Gaussian "
+            "parameters %.3f ms per 1M sources" % (
+                prec, res["relative_directions"],
+                res["gaussian_parameters"]))
+        del sky
+    lib.oskar_timer_free(timer)
+    ref = oracle.ref_sky()
+    if ref is not None:
+        ns = 100_000
+        st = C.c_int(0)
+        rsky = ref.oskar_sky_create(8, 0, ns, C.byref(st))
+        for t, v in cols.items():
+            p = ref.oskar_sky_column(rsky, t, 0, C.byref(st))
+            dst = np.ctypeslib.as_array(
+                C.cast(ref.oskar_mem_void(p), C.POINTER(C.c_double)), (ns,))
+            dst[:] = v[:ns]
+        nf = C.c_int(0)
+        t0 = time.time()
+        ref.oskar_sky_evaluate_gaussian_source_parameters(
+            rsky, 0, ra0, dec0, C.byref(nf), C.byref(st))
+        tr = time.time() - t0
+        ref.oskar_sky_free(rsky, C.byref(st))
+        ms_1m = 1e3 * tr * S / ns
+        out["reference_cpu"] = {
+            "gaussian_parameters_ms_per_1M": round(ms_1m, 2),
+            "cores": 1, "kind": "reference",
+            "sample": "first 100k sources of the FP64 sky, serial"}
+        out["speedup_gaussian_f64"] = round(
+            ms_1m / out["f64"]["gaussian_parameters_ms"], 1)
+        log("reference Gaussian parameters (CPU, serial): %.1f ms per 1M "
+            "sources" % ms_1m)
+    return out
 
 
 def bench_vis(args, rng):
