@@ -15,10 +15,14 @@
  *    second stations 2*(t%8) and 2*(t%8)+1, so J_first * B is formed once
  *    and reused for two baselines.
  *  - Sources are streamed in tiles of TILE sources.  The Jones values of the
- *    48 stations and the source parameters of a tile are loaded with
- *    coalesced global loads and stored transposed in LDS as
- *    [source][station], so the inner loop reads a first-station value that
- *    is broadcast to 8 lanes and 16 consecutive second-station values.
+ *    48 stations are loaded as 16-byte vectors, one station row of the tile
+ *    per wave and instruction (64 lanes on consecutive bytes), and stored
+ *    transposed in LDS as [source][station], so the inner loop reads a
+ *    first-station value that is broadcast to 8 lanes and 16 consecutive
+ *    second-station values.  Source parameters are stored as 4-value
+ *    vectors, one LDS read each.
+ *  - The second-station Jones is scaled by the smearing factor, and every
+ *    term of M J^H is one fused multiply-add into the tile sum.
  *  - Single precision: each tile is summed in FP32 registers, then folded
  *    into FP64 accumulators (cross-tile sums in FP64).  Double precision
  *    accumulates in FP64 throughout.
@@ -103,11 +107,28 @@ __device__ __forceinline__ FP smearing(const BaselineTerms<FP, TS>& t,
     return s;
 }
 
-/* Number of reals per Jones value and the source-tile length. */
+/* Number of reals per Jones value and the source-tile length.  A Jones
+ * value is moved as VPE vectors of VR reals (16 bytes, or 8 for a float
+ * complex scalar); TILE * VPE = 64, so one wave loads one station row of
+ * a tile with consecutive lanes on consecutive bytes. */
 template <typename FP, bool MATRIX> struct Geometry
 {
     static constexpr int NR = MATRIX ? 8 : 2;
     static constexpr int TILE = MATRIX ? (sizeof(FP) == 4 ? 32 : 16) : 64;
+    static constexpr int VPE = MATRIX ? (sizeof(FP) == 4 ? 2 : 4) : 1;
+    static constexpr int VR = NR / VPE;
+    static_assert(TILE * VPE == 64, "one wave per station row of a tile");
+};
+
+template <typename FP, int VR> struct alignas(VR * sizeof(FP)) Vec
+{
+    FP v[VR];
+};
+
+/* Four source values, read from LDS as one vector. */
+template <typename FP> struct alignas(4 * sizeof(FP)) Quad
+{
+    FP x, y, z, w;
 };
 
 template <typename FP, bool MATRIX, bool GAUSS, bool BS, bool TS>
@@ -117,10 +138,15 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
     using G = Geometry<FP, MATRIX>;
     constexpr int NR = G::NR;
     constexpr int TILE = G::TILE;
+    constexpr int VPE = G::VPE;
+    constexpr int VR = G::VR;
+    using V = Vec<FP, VR>;
     /* Padding of one station row spreads the transposed stores over banks. */
     __shared__ FP sJa[TILE][BLOCK_A + 1][NR];
     __shared__ FP sJb[TILE][BLOCK_B + 1][NR];
-    __shared__ FP sSrc[10][TILE];
+    __shared__ Quad<FP> sFlux[TILE];      /* I, Q, U, V */
+    __shared__ Quad<FP> sDir[TILE];       /* l, m, n - 1 */
+    __shared__ Quad<FP> sExt[TILE];       /* a, b, c */
 
     const int N = A.num_stations;
     const int a0 = blockIdx.x * BLOCK_A;
@@ -167,61 +193,59 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
         for (int r = 0; r < NR; ++r) acc[k][r] = 0.0;
 
     const FP* J = (const FP*) A.jones;
+    /* Tile loads: wave w moves station rows w, w + 4, ... of the block
+     * (rows 0-31 first stations, 32-47 second stations); lane (s, q) moves
+     * vector q of source s.  Whole-vector loads need the Jones array
+     * aligned to a vector, which element-aligned arrays are; otherwise the
+     * same mapping moves one real at a time. */
+    const int lane = tid & 63, wave = tid >> 6;
+    const int lq = lane % VPE, ls = lane / VPE;
+    const bool vec_ok = (((uintptr_t) J) % sizeof(V)) == 0;
     for (int s0 = s_begin; s0 < s_end; s0 += TILE)
     {
         const int ns = min(TILE, s_end - s0);
         __syncthreads();
-        /* Jones tiles: element (station, s) holds NR reals; NR/2 complex
-         * values are loaded per thread-step as FP pairs. */
-        for (int f = tid; f < (BLOCK_A + BLOCK_B) * TILE * (NR / 2);
-                f += THREADS)
+        const bool s_ok = ls < ns;
+#pragma unroll
+        for (int i = 0; i < (BLOCK_A + BLOCK_B) / 4; ++i)
         {
-            const int half = f % (NR / 2);
-            const int s = (f / (NR / 2)) % TILE;
-            const int st = f / ((NR / 2) * TILE);
-            const int station = st < BLOCK_A ? a0 + st : b0 + (st - BLOCK_A);
-            FP re = (FP) 0, im = (FP) 0;
-            if (station < N && s < ns)
+            const int st = wave + 4 * i;
+            const bool first = i < BLOCK_A / 4;
+            const int station = first ? a0 + st : b0 + (st - BLOCK_A);
+            FP* dst = first ? &sJa[ls][st][lq * VR] :
+                    &sJb[ls][st - BLOCK_A][lq * VR];
+            const FP* src = J + ((size_t) station * A.num_sources + s0 +
+                    ls) * NR + lq * VR;
+            V x;
+#pragma unroll
+            for (int r = 0; r < VR; ++r) x.v[r] = (FP) 0;
+            if (station < N && s_ok)
             {
-                const size_t g = ((size_t) station * A.num_sources + s0 + s)
-                        * NR + half * 2;
-                re = J[g];
-                im = J[g + 1];
+                if (vec_ok)
+                    x = *(const V*) src;
+                else
+                {
+#pragma unroll
+                    for (int r = 0; r < VR; ++r) x.v[r] = src[r];
+                }
             }
-            if (st < BLOCK_A)
-            {
-                sJa[s][st][half * 2] = re;
-                sJa[s][st][half * 2 + 1] = im;
-            }
-            else
-            {
-                sJb[s][st - BLOCK_A][half * 2] = re;
-                sJb[s][st - BLOCK_A][half * 2 + 1] = im;
-            }
+            *(V*) dst = x;
         }
         for (int s = tid; s < TILE; s += THREADS)
         {
             const bool in = s < ns;
             const int g = s0 + s;
-            sSrc[0][s] = in ? A.I[g] : (FP) 0;
-            if (MATRIX)
-            {
-                sSrc[1][s] = in ? A.Q[g] : (FP) 0;
-                sSrc[2][s] = in ? A.U[g] : (FP) 0;
-                sSrc[3][s] = in ? A.V[g] : (FP) 0;
-            }
+            const FP z = (FP) 0;
+            sFlux[s] = Quad<FP>{in ? A.I[g] : z,
+                    (MATRIX && in) ? A.Q[g] : z,
+                    (MATRIX && in) ? A.U[g] : z,
+                    (MATRIX && in) ? A.V[g] : z};
             if (BS || TS)
-            {
-                sSrc[4][s] = in ? A.l[g] : (FP) 0;
-                sSrc[5][s] = in ? A.m[g] : (FP) 0;
-                sSrc[6][s] = in ? A.n[g] - (FP) 1 : (FP) 0;
-            }
+                sDir[s] = Quad<FP>{in ? A.l[g] : z, in ? A.m[g] : z,
+                        in ? A.n[g] - (FP) 1 : z, z};
             if (GAUSS)
-            {
-                sSrc[7][s] = in ? A.a[g] : (FP) 0;
-                sSrc[8][s] = in ? A.b[g] : (FP) 0;
-                sSrc[9][s] = in ? A.c[g] : (FP) 0;
-            }
+                sExt[s] = Quad<FP>{in ? A.a[g] : z, in ? A.b[g] : z,
+                        in ? A.c[g] : z, z};
         }
         __syncthreads();
 
@@ -234,19 +258,23 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
 #pragma unroll 2
         for (int s = 0; s < TILE; ++s)
         {
-            const FP l = sSrc[4][s], m = sSrc[5][s], n1 = sSrc[6][s];
+            const Quad<FP> fl = sFlux[s];
             FP sm[2];
+            if (GAUSS || BS || TS)
+            {
+                const Quad<FP> d = (BS || TS) ? sDir[s] : Quad<FP>{};
+                const Quad<FP> e = GAUSS ? sExt[s] : Quad<FP>{};
 #pragma unroll
-            for (int k = 0; k < 2; ++k)
-                sm[k] = smearing<FP, GAUSS, BS, TS>(bt[k], l, m, n1,
-                        sSrc[7][s], sSrc[8][s], sSrc[9][s]);
+                for (int k = 0; k < 2; ++k)
+                    sm[k] = smearing<FP, GAUSS, BS, TS>(bt[k], d.x, d.y, d.z,
+                            e.x, e.y, e.z);
+            }
             const FP* ja = sJa[s][la];
             if (MATRIX)
             {
                 /* B = [[I+Q, U+iV], [U-iV, I-Q]] (define_correlate_utils.h:36-39);
                  * M = J_first * B (define_multiply.h:69-87). */
-                const FP Ii = sSrc[0][s], Qi = sSrc[1][s];
-                const FP Ui = sSrc[2][s], Vi = sSrc[3][s];
+                const FP Ii = fl.x, Qi = fl.y, Ui = fl.z, Vi = fl.w;
                 const FP ba = Ii + Qi, bd = Ii - Qi;
                 FP M[8];
                 /* M.a = a*ba + b*conj(U+iV) */
@@ -264,31 +292,43 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
 #pragma unroll
                 for (int k = 0; k < 2; ++k)
                 {
+                    /* t += smear * M * J_second^H (define_multiply.h:54-67):
+                     * (M J^H).a = M.a conj(J.a) + M.b conj(J.b), etc.  The
+                     * smearing factor scales J_second; each term is one
+                     * fused multiply-add into the tile sum. */
                     const FP* jb = sJb[s][lb + k];
-                    /* P = M * J_second^H (define_multiply.h:54-67):
-                     * P.a = M.a conj(b.a) + M.b conj(b.b), etc. */
-                    FP P[8];
-                    P[0] = M[0] * jb[0] + M[1] * jb[1] + M[2] * jb[2] + M[3] * jb[3];
-                    P[1] = M[1] * jb[0] - M[0] * jb[1] + M[3] * jb[2] - M[2] * jb[3];
-                    P[2] = M[0] * jb[4] + M[1] * jb[5] + M[2] * jb[6] + M[3] * jb[7];
-                    P[3] = M[1] * jb[4] - M[0] * jb[5] + M[3] * jb[6] - M[2] * jb[7];
-                    P[4] = M[4] * jb[0] + M[5] * jb[1] + M[6] * jb[2] + M[7] * jb[3];
-                    P[5] = M[5] * jb[0] - M[4] * jb[1] + M[7] * jb[2] - M[6] * jb[3];
-                    P[6] = M[4] * jb[4] + M[5] * jb[5] + M[6] * jb[6] + M[7] * jb[7];
-                    P[7] = M[5] * jb[4] - M[4] * jb[5] + M[7] * jb[6] - M[6] * jb[7];
+                    FP w[8];
 #pragma unroll
-                    for (int r = 0; r < 8; ++r) t[k][r] += P[r] * sm[k];
+                    for (int r = 0; r < 8; ++r)
+                        w[r] = (GAUSS || BS || TS) ? jb[r] * sm[k] : jb[r];
+                    FP* T = t[k];
+                    T[0] += M[0] * w[0]; T[0] += M[1] * w[1];
+                    T[0] += M[2] * w[2]; T[0] += M[3] * w[3];
+                    T[1] += M[1] * w[0]; T[1] -= M[0] * w[1];
+                    T[1] += M[3] * w[2]; T[1] -= M[2] * w[3];
+                    T[2] += M[0] * w[4]; T[2] += M[1] * w[5];
+                    T[2] += M[2] * w[6]; T[2] += M[3] * w[7];
+                    T[3] += M[1] * w[4]; T[3] -= M[0] * w[5];
+                    T[3] += M[3] * w[6]; T[3] -= M[2] * w[7];
+                    T[4] += M[4] * w[0]; T[4] += M[5] * w[1];
+                    T[4] += M[6] * w[2]; T[4] += M[7] * w[3];
+                    T[5] += M[5] * w[0]; T[5] -= M[4] * w[1];
+                    T[5] += M[7] * w[2]; T[5] -= M[6] * w[3];
+                    T[6] += M[4] * w[4]; T[6] += M[5] * w[5];
+                    T[6] += M[6] * w[6]; T[6] += M[7] * w[7];
+                    T[7] += M[5] * w[4]; T[7] -= M[4] * w[5];
+                    T[7] += M[7] * w[6]; T[7] -= M[6] * w[7];
                 }
             }
             else
             {
-                const FP Ii = sSrc[0][s];
+                const FP Ii = fl.x;
 #pragma unroll
                 for (int k = 0; k < 2; ++k)
                 {
                     const FP* jb = sJb[s][lb + k];
                     /* J_first * conj(J_second) * (smear * I) */
-                    const FP w = sm[k] * Ii;
+                    const FP w = (GAUSS || BS || TS) ? sm[k] * Ii : Ii;
                     const FP re = ja[0] * jb[0] + ja[1] * jb[1];
                     const FP im = ja[1] * jb[0] - ja[0] * jb[1];
                     t[k][0] += re * w;
