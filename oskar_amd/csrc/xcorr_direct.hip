@@ -12,8 +12,8 @@
  * Work decomposition (MI355X-specific, not the reference's):
  *  - A workgroup of 256 threads (four wave64s) owns a block of 32 "first"
  *    stations x 16 "second" stations.  Thread t owns first station t/8 and
- *    second stations 2*(t%8) and 2*(t%8)+1, so J_first * B is formed once
- *    and reused for two baselines.
+ *    second stations t%8 and t%8 + 8, so J_first * B is formed once and
+ *    reused for two baselines.
  *  - Sources are streamed in tiles of TILE sources.  The Jones values of the
  *    48 stations are loaded as 16-byte vectors, one station row of the tile
  *    per wave and instruction (64 lanes on consecutive bytes), and stored
@@ -160,9 +160,12 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
     }
     const int tid = threadIdx.x;
     const int la = tid >> 3;
-    const int lb = (tid & 7) * 2;
+    /* Second stations lb and lb + 8: the eight lanes of a wave that share
+     * a first station read eight consecutive second stations, which fall
+     * in distinct LDS banks. */
+    const int lb = tid & 7;
     const int sa = a0 + la;
-    const int sb[2] = { b0 + lb, b0 + lb + 1 };
+    const int sb[2] = { b0 + lb, b0 + lb + 8 };
 
     /* Baseline terms and validity for the two baselines of this thread:
      * in[k] = a real baseline in this block, ok[k] = also inside the uv
@@ -296,7 +299,7 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
                      * (M J^H).a = M.a conj(J.a) + M.b conj(J.b), etc.  The
                      * smearing factor scales J_second; each term is one
                      * fused multiply-add into the tile sum. */
-                    const FP* jb = sJb[s][lb + k];
+                    const FP* jb = sJb[s][lb + 8 * k];
                     FP w[8];
 #pragma unroll
                     for (int r = 0; r < 8; ++r)
@@ -326,7 +329,7 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
 #pragma unroll
                 for (int k = 0; k < 2; ++k)
                 {
-                    const FP* jb = sJb[s][lb + k];
+                    const FP* jb = sJb[s][lb + 8 * k];
                     /* J_first * conj(J_second) * (smear * I) */
                     const FP w = (GAUSS || BS || TS) ? sm[k] * Ii : Ii;
                     const FP re = ja[0] * jb[0] + ja[1] * jb[1];
