@@ -122,6 +122,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--clip-steps", type=int, default=10,
                     help="horizon clips of a 1M-source catalogue (0 = skip)")
+    ap.add_argument("--direct-channels", type=int, default=16,
+                    help="channels of the 512 x 100k Gaussian + smearing "
+                         "direct-correlator leg (0 = skip)")
     ap.add_argument("--sky-steps", type=int, default=5,
                     help="relative directions and Gaussian parameters of a "
                          "1M-source sky (0 = skip)")
@@ -420,6 +423,9 @@ def main():
     sky_params = None
     if args.sky_steps > 0:
         sky_params = bench_sky_params(args, rng)
+    direct = None
+    if args.direct_channels > 0:
+        direct = bench_direct_smearing(args, rng)
 
     # --- batched (channel, time) units in one persistent launch ---------
     batch = None
@@ -514,6 +520,7 @@ def main():
         "jones_K_join": jones,
         "horizon_clip": clip,
         "sky_parameters": sky_params,
+        "direct_gaussian_smearing": direct,
         "vis_block": vis,
     }
     print(json.dumps(line), flush=True)
@@ -590,6 +597,141 @@ def bench_clip(args, rng):
             "achieved_GBps": round(gbps, 1),
             "hbm_peak_GBps": PEAK_HBM_GBPS,
             "frac": round(gbps / PEAK_HBM_GBPS, 4)}
+
+
+def bench_direct_smearing(args, rng):
+    """oskar_cross_correlate (the direct kernel) on 512 stations x 100,000
+    sources, 30 % Gaussian (FWHM 5-60 arcsec, a, b, c from the library's
+    oskar_sky_evaluate_gaussian_source_parameters), full Stokes, bandwidth
+    smearing (10 kHz) and time smearing (1 s), RADEC phase centre, in FP32
+    and FP64, one call per channel over `direct_channels` channels in
+    50-350 MHz (log-radial layout to 35 km; Jones = CN(0, 1) + I per
+    station and source).  Work per evaluation: 133 flops + 2 sines + 2
+    divisions + 1 exponential.  The reference's own
+    oskar_cross_correlate_gaussian_omp_d (oracle/_ref) runs the same
+    configuration on the host for a sample of the sources."""
+    import oracle
+    from oskar_amd import capi, lib
+    from oskar_amd.mem import (OSKAR_COMPLEX, OSKAR_DOUBLE, OSKAR_GPU,
+                               OSKAR_MATRIX, OSKAR_SINGLE, Jones, Mem, Sky,
+                               Telescope, check_status)
+    N, S = 512, 100_000
+    NB = N * (N - 1) // 2
+    ra0, dec0, gast = 0.4, -0.5, 1.3
+    bw, tavg = 10e3, 1.0
+    freqs = np.linspace(50e6, 350e6, args.direct_channels)
+    r = np.exp(rng.uniform(np.log(100.0), np.log(35000.0), N))
+    th = rng.uniform(0, 2 * np.pi, N)
+    uvw = [r * np.cos(th), r * np.sin(th), rng.uniform(-50, 50, N)]
+    rho = np.radians(5.0) * np.sqrt(rng.uniform(0, 1, S))
+    phi = rng.uniform(0, 2 * np.pi, S)
+    l, m = np.sin(rho) * np.cos(phi), np.sin(rho) * np.sin(phi)
+    nn = np.sqrt(1 - l * l - m * m)
+    dec = np.arcsin(nn * np.sin(dec0) + m * np.cos(dec0))
+    ra = ra0 + np.arctan2(l, nn * np.cos(dec0) - m * np.sin(dec0))
+    I_ = power_law(rng, S)
+    pol = rng.uniform(0, 0.2, S) * I_
+    d = rng.normal(size=(3, S))
+    d /= np.linalg.norm(d, axis=0)
+    gauss = rng.uniform(0, 1, S) < 0.3
+    maj = np.where(gauss, np.radians(rng.uniform(5, 60, S) / 3600), 0.0)
+    mn = maj * rng.uniform(0.2, 1.0, S)
+    pa = rng.uniform(0, np.pi, S)
+    cols = {1: ra, 2: dec, 3: I_, 4: pol * d[0], 5: pol * d[1],
+            6: pol * d[2], 10: maj, 11: mn, 12: pa}
+    J = rng.standard_normal(N * S * 8, dtype=np.float32)
+    J *= np.float32(1 / np.sqrt(2))
+    J = J.reshape(N, S, 8)
+    J[:, :, 0] += 1
+    J[:, :, 6] += 1
+    status = C.c_int(0)
+    timer = lib.oskar_timer_create(1)
+    out = {"workload": "512 stations x 100k sources (30 %% Gaussian, full "
+                       "Stokes), bandwidth (10 kHz) and time (1 s) "
+                       "smearing, %d channels in 50-350 MHz" % len(freqs)}
+    evals = float(NB) * S * len(freqs)
+    for prec, code in (("f32", OSKAR_SINGLE), ("f64", OSKAR_DOUBLE)):
+        sky = Sky(code, OSKAR_GPU, S)
+        for t, v in cols.items():
+            sky.set_column(t, v)
+        nf = C.c_int(0)
+        lib.oskar_sky_evaluate_relative_directions(sky.ptr, ra0, dec0,
+                                                   C.byref(status))
+        lib.oskar_sky_evaluate_gaussian_source_parameters(
+            sky.ptr, 0, ra0, dec0, C.byref(nf), C.byref(status))
+        jt = code | OSKAR_COMPLEX | OSKAR_MATRIX
+        jones = Jones(jt, OSKAR_GPU, N, S)
+        jones.set(J if code == OSKAR_SINGLE else J.astype(np.float64))
+        tel = Telescope(code, OSKAR_GPU, N)
+        tel.set_station_ecef(uvw[0], uvw[1], np.zeros(N))
+        lib.oskar_telescope_set_channel_bandwidth(tel.ptr, bw)
+        lib.oskar_telescope_set_time_average(tel.ptr, tavg)
+        lib.oskar_telescope_set_phase_centre(tel.ptr, 2, ra0, dec0)
+        um = [Mem.from_numpy(x, code) for x in uvw]
+        vis = Mem.from_numpy(np.zeros(NB * 8), jt)
+        fl = capi.MEMA(*[sky.column(t).ptr for t in (3, 4, 5, 6)])
+        dr = capi.MEM3(*[sky.column(t).ptr for t in (0x103, 0x104, 0x105)])
+        ex = capi.MEM3(*[sky.column(t).ptr for t in (0x100, 0x101, 0x102)])
+        uv = capi.MEM3(*[x.ptr for x in um])
+
+        def one(f):
+            lib.oskar_cross_correlate(0, 1, S, jones.ptr, fl, dr, ex, tel.ptr,
+                                      uv, gast, float(f), 0, vis.ptr,
+                                      C.byref(status))
+
+        one(freqs[0])                       # warm-up
+        lib.oskar_device_synchronize(OSKAR_GPU)
+        lib.oskar_timer_start(timer)
+        for f in freqs:
+            one(f)
+        lib.oskar_device_synchronize(OSKAR_GPU)
+        t = lib.oskar_timer_elapsed(timer)
+        check_status(status, "direct correlator (%s)" % prec)
+        rate = evals / t
+        out[prec] = {"ms_per_channel": round(1e3 * t / len(freqs), 3),
+                     "evals_per_s": rate,
+                     "achieved_TFLOPs_133_per_eval": round(rate * 133e-12,
+                                                           2)}
+        log("direct Gaussian + smearing %s: %.1f ms per channel, %.3e "
+            "evals/s" % (prec, 1e3 * t / len(freqs), rate))
+        del jones, sky, vis
+    lib.oskar_timer_free(timer)
+    out["fp32_vector_peak_TFLOPs"] = 157.3
+    ref = oracle.ref()
+    if ref is not None:
+        dirs = oracle.relative_directions("d", ra, dec, ra0, dec0)
+        abc = oracle.gaussian_parameters("d", ra, dec, maj, mn, pa, ra0,
+                                         dec0)[:3]
+        ns = 2000
+        f = float(freqs[-1])
+        ha0 = gast - ra0
+        P = lambda a: np.ascontiguousarray(a, np.float64).ctypes.data_as(
+            C.c_void_p)
+        keep = [np.ascontiguousarray(J[:, :ns].astype(np.float64))]
+        arrs = [cols[k][:ns] for k in (3, 4, 5, 6)] + \
+            [x[:ns] for x in dirs] + [x[:ns] for x in abc] + uvw + \
+            [uvw[0], uvw[1]]
+        arrs = [np.ascontiguousarray(a, np.float64) for a in arrs]
+        vout = np.zeros(NB * 8)
+        D = C.c_double
+        t0 = time.time()
+        ref.oskar_cross_correlate_gaussian_omp_d(
+            C.c_int(0), C.c_int(ns), C.c_int(N), C.c_int(0),
+            keep[0].ctypes.data_as(C.c_void_p),
+            *[a.ctypes.data_as(C.c_void_p) for a in arrs],
+            D(0.0), D(3.4e38), D(f / C0), D(bw / f), D(tavg), D(ha0),
+            D(dec0), vout.ctypes.data_as(C.c_void_p))
+        tc = time.time() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["reference_cpu"] = {
+            "evals_per_s": NB * ns / tc, "cores": cores,
+            "kind": "reference",
+            "sample": "512 stations x %d sources, one channel, FP64" % ns}
+        out["speedup_f32_vs_reference_cpu"] = round(
+            out["f32"]["evals_per_s"] / (NB * ns / tc), 1)
+        log("reference direct Gaussian + smearing (CPU, %d cores): %.3e "
+            "evals/s" % (cores, NB * ns / tc))
+    return out
 
 
 def bench_sky_params(args, rng):

@@ -258,7 +258,10 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
 #pragma unroll
             for (int r = 0; r < NR; ++r) t[k][r] = (FP) 0;
 
-#pragma unroll 2
+        /* Two sources per iteration in FP32; one in FP64, whose registers
+         * would otherwise limit the kernel to two waves per SIMD. */
+        constexpr int UNROLL = sizeof(FP) == 4 ? 2 : 1;
+#pragma unroll UNROLL
         for (int s = 0; s < TILE; ++s)
         {
             const Quad<FP> fl = sFlux[s];
