@@ -190,9 +190,29 @@ OSKAR_EXPORT oskar_Mem* oskar_station_work_horizon_mask(
 OSKAR_EXPORT oskar_Mem* oskar_station_work_source_indices(
         oskar_StationWork* work);
 
+/* ---- Permitted beam directions ----------------------------------------- */
+/* oskar/telescope/station/src/oskar_station_accessors.c:455-470: the number
+ * of permitted (azimuth, elevation) beam directions (0: any direction) and
+ * their values in radians (double, host memory; NULL before a load). */
+OSKAR_EXPORT int oskar_station_num_permitted_beams(const oskar_Station* model);
+OSKAR_EXPORT const oskar_Mem* oskar_station_permitted_beam_az_rad_const(
+        const oskar_Station* model);
+OSKAR_EXPORT const oskar_Mem* oskar_station_permitted_beam_el_rad_const(
+        const oskar_Station* model);
+
+/* oskar/telescope/station/src/oskar_station_load_permitted_beams.c: one
+ * direction per line, azimuth and elevation in degrees as the first two
+ * numbers (separated by commas, spaces or tabs; '#' starts a comment; lines
+ * with fewer than two numbers are skipped).  Replaces any loaded set.
+ * OSKAR_ERR_FILE_IO if the file cannot be opened. */
+OSKAR_EXPORT void oskar_station_load_permitted_beams(oskar_Station* station,
+        const char* filename, int* status);
+
 /* ---- Station-beam evaluation ------------------------------------------- */
 /* oskar/telescope/station/oskar_station_beam_horizon_direction.h:
- * beam direction cosines (RADEC via the apparent-RA/Dec formula, or AZEL). */
+ * beam direction cosines (RADEC via the apparent-RA/Dec formula, or AZEL);
+ * with permitted beams, the permitted (az, el) nearest on the sphere
+ * (haversine distance, first of equals) replaces the direction. */
 OSKAR_EXPORT
 void oskar_station_beam_horizon_direction(const oskar_Station* station,
         const double gast_rad, double* x, double* y, double* z, int* status);

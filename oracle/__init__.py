@@ -66,6 +66,11 @@ def lib():
         _lib.oracle_gaussian_parameters_f.argtypes = gp
         _lib.oracle_gaussian_parameters_d.restype = I
         _lib.oracle_gaussian_parameters_f.restype = I
+        _lib.oracle_angular_distance.argtypes = [D, D, D, D]
+        _lib.oracle_angular_distance.restype = D
+        DP = C.POINTER(C.c_double)
+        _lib.oracle_snap_beam_direction.argtypes = [DP, DP, DP, I, P, P]
+        _lib.oracle_snap_beam_direction.restype = I
     return _lib
 
 
@@ -364,6 +369,7 @@ def ref_sky():
             "oskar_mem_void": (P, [P]),
             "oskar_sky_evaluate_gaussian_source_parameters": (
                 None, [P, I, D, D, IP, IP]),
+            "oskar_angular_distance": (D, [D, D, D, D]),
             "ref_lon_lat_to_relative_directions_3d_float": (
                 None, [I, P, P, C.c_float, C.c_float, C.c_float, P, P, P]),
             "ref_lon_lat_to_relative_directions_3d_double": (
@@ -375,3 +381,14 @@ def ref_sky():
             f.argtypes = args
         _ref_sky = L
     return _ref_sky
+
+
+def snap_beam_direction(xyz, az, el):
+    """(x, y, z) snapped to the nearest permitted (az, el); returns
+    ((x, y, z), index)."""
+    x, y, z = (C.c_double(float(v)) for v in xyz)
+    az = _c(az, np.float64)
+    el = _c(el, np.float64)
+    k = lib().oracle_snap_beam_direction(C.byref(x), C.byref(y), C.byref(z),
+                                         az.size, _p(az), _p(el))
+    return (x.value, y.value, z.value), k

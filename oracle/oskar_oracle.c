@@ -992,3 +992,44 @@ GAUSS_PARAMS_BODY(oracle_gaussian_parameters_d, double, DBL_MIN, sin, cos,
         atan2, sqrt, asin)
 GAUSS_PARAMS_BODY(oracle_gaussian_parameters_f, float, FLT_MIN, sinf, cosf,
         atan2f, sqrtf, asinf)
+
+/* Great-circle distance (haversine; oskar/math/src/oskar_angular_distance.c
+ * :13-21). */
+double oracle_angular_distance(double lon1_rad, double lon2_rad,
+        double lat1_rad, double lat2_rad)
+{
+    const double sin_delta_lat = sin(0.5 * (lat2_rad - lat1_rad));
+    const double sin_delta_lon = sin(0.5 * (lon2_rad - lon1_rad));
+    return 2.0 * asin(sqrt(sin_delta_lat * sin_delta_lat +
+            cos(lat1_rad) * cos(lat2_rad) * sin_delta_lon * sin_delta_lon));
+}
+
+/* Snap of a beam direction to the nearest permitted (az, el)
+ * (oskar/telescope/station/src/oskar_station_beam_horizon_direction.c
+ * :50-88): direction cosines -> az, el; the permitted direction at the
+ * smallest haversine distance (the first of equals); back to direction
+ * cosines.  Returns its index. */
+int oracle_snap_beam_direction(double* x, double* y, double* z, int n,
+        const double* p_az, const double* p_el)
+{
+    int i, best = 0;
+    double best_dist = DBL_MAX;
+    const double az = atan2(*x, *y);
+    const double el = atan2(*z, sqrt(*x * *x + *y * *y));
+    for (i = 0; i < n; ++i)
+    {
+        const double d = oracle_angular_distance(p_az[i], az, p_el[i], el);
+        if (d < best_dist)
+        {
+            best_dist = d;
+            best = i;
+        }
+    }
+    {
+        const double cos_el = cos(p_el[best]);
+        *x = cos_el * sin(p_az[best]);
+        *y = cos_el * cos(p_az[best]);
+        *z = sin(p_el[best]);
+    }
+    return best;
+}

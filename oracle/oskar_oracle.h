@@ -174,6 +174,16 @@ int oracle_gaussian_parameters_f(int num, const float* ra, const float* dec,
         const float* maj, const float* min, const float* pa, double ra0,
         double dec0, float* a, float* b, float* c, int* failed);
 
+/* Great-circle distance by the haversine formula
+ * (oskar/math/src/oskar_angular_distance.c:13-21). */
+double oracle_angular_distance(double lon1_rad, double lon2_rad,
+        double lat1_rad, double lat2_rad);
+
+/* Nearest permitted (az, el) beam direction, replacing (x, y, z)
+ * (oskar_station_beam_horizon_direction.c:50-88); returns its index. */
+int oracle_snap_beam_direction(double* x, double* y, double* z, int n,
+        const double* p_az, const double* p_el);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,9 @@ struct oskar_Station
     oskar_Mem* element_types;     /* int, station location */
     oskar_Mem* element_types_cpu; /* int, host */
     oskar_Element** element;
+    int num_permitted_beams;          /* private_station.h:91-93 */
+    oskar_Mem* permitted_beam_az_rad; /* double, host; NULL until loaded */
+    oskar_Mem* permitted_beam_el_rad;
 };
 
 /* oskar/telescope/station/private_station_work.h (subset). */

@@ -17,7 +17,9 @@
  *  - oskar/telescope/station/src/oskar_station_beam_horizon_direction.c:17-90;
  *  - oskar/convert/define_convert_apparent_ra_dec_to_enu_directions.h.
  */
+#include <float.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -144,6 +146,8 @@ void oskar_station_free(oskar_Station* s, int* status)
                 status);
     oskar_mem_free(s->element_types, status);
     oskar_mem_free(s->element_types_cpu, status);
+    oskar_mem_free(s->permitted_beam_az_rad, status);
+    oskar_mem_free(s->permitted_beam_el_rad, status);
     for (i = 0; i < s->num_element_types; ++i) free(s->element[i]);
     free(s->element);
     free(s);
@@ -224,6 +228,14 @@ oskar_Station* oskar_station_create_copy(const oskar_Station* src,
     oskar_station_resize_element_types(s, src->num_element_types, status);
     for (i = 0; i < src->num_element_types && !*status; ++i)
         *s->element[i] = *src->element[i];
+    if (src->num_permitted_beams > 0 && !*status)
+    {
+        s->permitted_beam_az_rad = oskar_mem_create_copy(
+                src->permitted_beam_az_rad, OSKAR_CPU, status);
+        s->permitted_beam_el_rad = oskar_mem_create_copy(
+                src->permitted_beam_el_rad, OSKAR_CPU, status);
+        s->num_permitted_beams = src->num_permitted_beams;
+    }
     return s;
 }
 
@@ -509,6 +521,17 @@ void oskar_station_work_free(oskar_StationWork* w, int* status)
 }
 
 /* ---- Station-beam chain ------------------------------------------------- */
+/* Great-circle distance by the haversine formula
+ * (oskar/math/src/oskar_angular_distance.c:13-21). */
+static double angular_distance(double lon1_rad, double lon2_rad,
+        double lat1_rad, double lat2_rad)
+{
+    const double sin_dlat = sin(0.5 * (lat2_rad - lat1_rad));
+    const double sin_dlon = sin(0.5 * (lon2_rad - lon1_rad));
+    return 2.0 * asin(sqrt(sin_dlat * sin_dlat +
+            cos(lat1_rad) * cos(lat2_rad) * sin_dlon * sin_dlon));
+}
+
 void oskar_station_beam_horizon_direction(const oskar_Station* s,
         const double gast_rad, double* x, double* y, double* z, int* status)
 {
@@ -533,6 +556,134 @@ void oskar_station_beam_horizon_direction(const oskar_Station* s,
         *z = sin(s->beam_lat_rad);
     }
     else *status = OSKAR_ERR_INVALID_ARGUMENT;
+
+    /* Snap to the nearest permitted (az, el), by great-circle distance, the
+     * first of equal distances winning (:50-88). */
+    if (s->num_permitted_beams > 0)
+    {
+        int i = 0, best = 0;
+        double best_dist = DBL_MAX;
+        const double az = atan2(*x, *y);
+        const double el = atan2(*z, sqrt(*x * *x + *y * *y));
+        const double* p_az = (const double*)
+                oskar_mem_void_const(s->permitted_beam_az_rad);
+        const double* p_el = (const double*)
+                oskar_mem_void_const(s->permitted_beam_el_rad);
+        for (i = 0; i < s->num_permitted_beams; ++i)
+        {
+            const double d = angular_distance(p_az[i], az, p_el[i], el);
+            if (d < best_dist)
+            {
+                best_dist = d;
+                best = i;
+            }
+        }
+        {
+            const double cos_el = cos(p_el[best]);
+            *x = cos_el * sin(p_az[best]);
+            *y = cos_el * cos(p_az[best]);
+            *z = sin(p_el[best]);
+        }
+    }
+}
+
+int oskar_station_num_permitted_beams(const oskar_Station* model)
+{
+    return model ? model->num_permitted_beams : 0;
+}
+
+const oskar_Mem* oskar_station_permitted_beam_az_rad_const(
+        const oskar_Station* model)
+{
+    return model ? model->permitted_beam_az_rad : 0;
+}
+
+const oskar_Mem* oskar_station_permitted_beam_el_rad_const(
+        const oskar_Station* model)
+{
+    return model ? model->permitted_beam_el_rad : 0;
+}
+
+/* The first two numbers of a line (separators ',', ' ', '\t'; a token
+ * starting with '#' ends the line), as oskar_string_to_array_d reads them
+ * (oskar/utility/src/oskar_string_to_array.c). */
+static int two_numbers(char* line, double* v)
+{
+    int n = 0;
+    char* save = 0;
+    char* tok = strtok_r(line, ", \t\r\n", &save);
+    while (tok && n < 2)
+    {
+        char* end = 0;
+        if (tok[0] == '#') break;
+        const double x = strtod(tok, &end);
+        if (end > tok) v[n++] = x;
+        tok = strtok_r(0, ", \t\r\n", &save);
+    }
+    return n;
+}
+
+void oskar_station_load_permitted_beams(oskar_Station* station,
+        const char* filename, int* status)
+{
+    int n = 0, cap = 0;
+    double *az = 0, *el = 0;
+    char line[4096];
+    FILE* file = 0;
+    if (*status || !station) return;
+    if (station->precision != OSKAR_SINGLE &&
+            station->precision != OSKAR_DOUBLE)
+    {
+        *status = OSKAR_ERR_BAD_DATA_TYPE;
+        return;
+    }
+    file = fopen(filename, "r");
+    if (!file)
+    {
+        *status = OSKAR_ERR_FILE_IO;
+        return;
+    }
+    while (fgets(line, sizeof(line), file))
+    {
+        double par[2] = {0.0, 0.0};
+        if (two_numbers(line, par) < 2) continue;
+        if (n == cap)
+        {
+            cap += 100;
+            double* a = (double*) realloc(az, (size_t) cap * sizeof(double));
+            if (a) az = a;
+            double* e = (double*) realloc(el, (size_t) cap * sizeof(double));
+            if (e) el = e;
+            if (!a || !e)
+            {
+                *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE;
+                break;
+            }
+        }
+        az[n] = par[0] * M_PI / 180.0;
+        el[n] = par[1] * M_PI / 180.0;
+        ++n;
+    }
+    fclose(file);
+    if (!*status)
+    {
+        oskar_mem_free(station->permitted_beam_az_rad, status);
+        oskar_mem_free(station->permitted_beam_el_rad, status);
+        station->permitted_beam_az_rad = oskar_mem_create(OSKAR_DOUBLE,
+                OSKAR_CPU, (size_t) n, status);
+        station->permitted_beam_el_rad = oskar_mem_create(OSKAR_DOUBLE,
+                OSKAR_CPU, (size_t) n, status);
+        if (!*status && n > 0)
+        {
+            memcpy(oskar_mem_void(station->permitted_beam_az_rad), az,
+                    (size_t) n * sizeof(double));
+            memcpy(oskar_mem_void(station->permitted_beam_el_rad), el,
+                    (size_t) n * sizeof(double));
+        }
+        station->num_permitted_beams = *status ? 0 : n;
+    }
+    free(az);
+    free(el);
 }
 
 static int check_gpu(const oskar_Mem* m, int* status)
