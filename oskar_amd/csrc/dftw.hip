@@ -17,8 +17,15 @@
  *    reads the element pattern once per direction instead of once per
  *    element; with data_idx absent each element is its own run and the
  *    kernel reads data per element as the reference does.
- *  - Single precision uses the hardware sine/cosine (__sincosf) of the
- *    FP32 phase; double precision uses the full-accuracy sincos.
+ *  - Elements are staged with position and weight in one LDS record (a
+ *    broadcast read per element).  A chunk whose elements all use one
+ *    data row (the usual case: one element type, or long runs of one)
+ *    skips the per-element run check and unrolls the element loop by 4.
+ *  - Single precision carries the phase in revolutions (the wavenumber
+ *    times 1/(2 pi) folded into the direction once per thread), reduces it
+ *    to [-1/2, 1/2] and uses the hardware sine and cosine, which take
+ *    revolutions; each term is four fused multiply-adds into the array
+ *    factor.  Double precision uses radians and the full-accuracy sincos.
  */
 #include <hip/hip_runtime.h>
 
@@ -32,12 +39,35 @@ namespace {
 constexpr int THREADS = 256;
 constexpr int CHUNK = 256;
 
-template <typename FP> __device__ __forceinline__ void fast_sincos(FP t,
+/* exp(i t) of the phase in the unit the kernel carries it in: FP32 phases
+ * are in revolutions, reduced to [-1/2, 1/2] and fed to the hardware sine
+ * and cosine (which take revolutions); FP64 phases are in radians, with the
+ * full-accuracy sincos. */
+template <typename FP> __device__ __forceinline__ void phase_sincos(FP t,
         FP* s, FP* c);
-template <> __device__ __forceinline__ void fast_sincos<float>(float t,
-        float* s, float* c) { __sincosf(t, s, c); }
-template <> __device__ __forceinline__ void fast_sincos<double>(double t,
+template <> __device__ __forceinline__ void phase_sincos<float>(float t,
+        float* s, float* c)
+{
+    const float r = t - __builtin_rintf(t);
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
+}
+template <> __device__ __forceinline__ void phase_sincos<double>(double t,
         double* s, double* c) { sincos(t, s, c); }
+
+/* Factor from k (x_out x_in + ...) to the kernel's phase unit. */
+template <typename FP> __device__ __forceinline__ FP phase_unit();
+template <> __device__ __forceinline__ float phase_unit<float>()
+{ return 0.159154943091895336f; }           /* 1 / (2 pi) */
+template <> __device__ __forceinline__ double phase_unit<double>()
+{ return 1.0; }
+
+/* One element of the chunk staged in LDS: position and weight. */
+template <typename FP> struct Elem
+{
+    FP x, y, z, pad;
+    Cplx<FP> w;
+};
 
 template <typename FP, bool MATRIX, bool IS_3D>
 __global__ __launch_bounds__(THREADS)
@@ -52,18 +82,19 @@ void dftw_kernel(const int num_in, const FP wavenumber,
         const int offset_out, Cplx<FP>* __restrict__ output,
         const FP norm_factor)
 {
-    __shared__ Cplx<FP> c_w[CHUNK];
-    __shared__ FP c_x[CHUNK], c_y[CHUNK], c_z[CHUNK];
+    __shared__ Elem<FP> c_e[CHUNK];
     __shared__ int c_idx[CHUNK];
+    __shared__ int c_uniform;
 
     const int i_out = blockIdx.x * THREADS + threadIdx.x;
     const bool active = i_out < num_out;
     FP xo = (FP) 0, yo = (FP) 0, zo = (FP) 0;
     if (active)
     {
-        xo = wavenumber * x_out[i_out + offset_coord_out];
-        yo = wavenumber * y_out[i_out + offset_coord_out];
-        if (IS_3D) zo = wavenumber * z_out[i_out + offset_coord_out];
+        const FP kk = wavenumber * phase_unit<FP>();
+        xo = kk * x_out[i_out + offset_coord_out];
+        yo = kk * y_out[i_out + offset_coord_out];
+        if (IS_3D) zo = kk * z_out[i_out + offset_coord_out];
     }
     constexpr int NC = MATRIX ? 4 : 1;
     Cplx<FP> out[NC];
@@ -90,37 +121,71 @@ void dftw_kernel(const int num_in, const FP wavenumber,
         }
         af.x = af.y = (FP) 0;
     };
+    /* af += w exp(i t) for element e: four fused multiply-adds. */
+    auto term = [&](const Elem<FP>& e) {
+        FP t = xo * e.x + yo * e.y;
+        if (IS_3D) t += zo * e.z;
+        FP s, c;
+        phase_sincos<FP>(t, &s, &c);
+        af.x += e.w.x * c;
+        af.x -= e.w.y * s;
+        af.y += e.w.x * s;
+        af.y += e.w.y * c;
+    };
 
     for (int j = 0; j < num_in; j += CHUNK)
     {
         const int chunk = min(CHUNK, num_in - j);
         __syncthreads();
+        if (threadIdx.x == 0) c_uniform = 1;
+        __syncthreads();
         for (int t = threadIdx.x; t < chunk; t += THREADS)
         {
             const int g = j + t;
-            c_w[t] = weights[g];
-            c_x[t] = x_in[g];
-            c_y[t] = y_in[g];
-            if (IS_3D) c_z[t] = z_in[g];
-            c_idx[t] = data_idx ? data_idx[g] : g;
+            Elem<FP> e;
+            e.x = x_in[g];
+            e.y = y_in[g];
+            e.z = IS_3D ? z_in[g] : (FP) 0;
+            e.pad = (FP) 0;
+            e.w = weights[g];
+            c_e[t] = e;
+            const int idx = data_idx ? data_idx[g] : g;
+            c_idx[t] = idx;
+            /* Any element whose data row differs from the chunk's first
+             * clears the flag (every writer stores the same value). */
+            if (idx != (data_idx ? data_idx[j] : j)) c_uniform = 0;
         }
         __syncthreads();
-        for (int i = 0; i < chunk; ++i)
+        if (c_uniform)
         {
-            const int idx = c_idx[i];
+            /* One data row for the whole chunk: no run checks, and the
+             * element loop unrolled. */
+            const int idx = c_idx[0];
             if (idx != run_idx)
             {
                 flush();
                 run_idx = idx;
             }
-            FP t = xo * c_x[i] + yo * c_y[i];
-            if (IS_3D) t += zo * c_z[i];
-            FP s, c;
-            fast_sincos<FP>(t, &s, &c);
-            const Cplx<FP> w = c_w[i];
-            /* af += w * exp(i t) */
-            af.x += w.x * c - w.y * s;
-            af.y += w.x * s + w.y * c;
+            int i = 0;
+            for (; i + 4 <= chunk; i += 4)
+            {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) term(c_e[i + u]);
+            }
+            for (; i < chunk; ++i) term(c_e[i]);
+        }
+        else
+        {
+            for (int i = 0; i < chunk; ++i)
+            {
+                const int idx = c_idx[i];
+                if (idx != run_idx)
+                {
+                    flush();
+                    run_idx = idx;
+                }
+                term(c_e[i]);
+            }
         }
     }
     flush();
