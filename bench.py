@@ -122,6 +122,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--clip-steps", type=int, default=10,
                     help="horizon clips of a 1M-source catalogue (0 = skip)")
+    ap.add_argument("--dftw-steps", type=int, default=5,
+                    help="oskar_dftw calls at 4096 elements x 1M directions "
+                         "(0 = skip)")
     ap.add_argument("--direct-channels", type=int, default=16,
                     help="channels of the 512 x 100k Gaussian + smearing "
                          "direct-correlator leg (0 = skip)")
@@ -426,6 +429,9 @@ def main():
     direct = None
     if args.direct_channels > 0:
         direct = bench_direct_smearing(args, rng)
+    dftw = None
+    if args.dftw_steps > 0:
+        dftw = bench_dftw(args, rng)
 
     # --- batched (channel, time) units in one persistent launch ---------
     batch = None
@@ -521,6 +527,7 @@ def main():
         "horizon_clip": clip,
         "sky_parameters": sky_params,
         "direct_gaussian_smearing": direct,
+        "dftw": dftw,
         "vis_block": vis,
     }
     print(json.dumps(line), flush=True)
@@ -597,6 +604,82 @@ def bench_clip(args, rng):
             "achieved_GBps": round(gbps, 1),
             "hbm_peak_GBps": PEAK_HBM_GBPS,
             "frac": round(gbps / PEAK_HBM_GBPS, 4)}
+
+
+def bench_dftw(args, rng):
+    """oskar_dftw alone: 4096 elements (76 m disc, planar) x 1M directions
+    over the upper hemisphere, FP32, one 2x2 element pattern per direction
+    (one element type), 150 MHz.  Unit: one (element, direction) term.  The
+    kernel's inner loop issues 8 VALU instructions (phase, its reduction,
+    four fused multiply-adds) and 2 transcendental ones (sine and cosine)
+    per term; at 4 cycles per wave64 instruction that bounds it at
+    256 CUs x 4 SIMDs x 64 lanes x 2.4 GHz / 40 = 3.93e12 terms/s, the
+    roofline reported.  The oracle's OpenMP body (the reference's CPU
+    arithmetic) runs a 20k-direction sample on the host cores."""
+    import oracle
+    from oskar_amd import lib
+    from oskar_amd.mem import (OSKAR_COMPLEX, OSKAR_GPU, OSKAR_INT,
+                               OSKAR_MATRIX, OSKAR_SINGLE, Mem, check_status)
+    ne, nd = 4096, 1_000_000
+    f = 150e6
+    k = 2 * np.pi * f / C0
+    r = 76.0 * np.sqrt(rng.uniform(size=ne))
+    th = rng.uniform(0, 2 * np.pi, ne)
+    x_in, y_in = r * np.cos(th), r * np.sin(th)
+    w = np.exp(-1j * rng.uniform(0, 2 * np.pi, ne))
+    wr = np.stack([w.real, w.imag], -1).ravel()
+    nz = rng.uniform(0, 1, nd)
+    ph = rng.uniform(0, 2 * np.pi, nd)
+    xo = np.sqrt(1 - nz * nz) * np.sin(ph)
+    yo = np.sqrt(1 - nz * nz) * np.cos(ph)
+    data = rng.normal(size=nd * 8).astype(np.float32)
+    CT = OSKAR_SINGLE | OSKAR_COMPLEX
+    mw = Mem.from_numpy(wr, CT)
+    mx, my = (Mem.from_numpy(a, OSKAR_SINGLE) for a in (x_in, y_in))
+    ox, oy = (Mem.from_numpy(a, OSKAR_SINGLE) for a in (xo, yo))
+    midx = Mem.from_numpy(np.zeros(ne, np.int32), OSKAR_INT)
+    md = Mem.from_numpy(data, CT | OSKAR_MATRIX)
+    out = Mem.from_numpy(np.zeros(nd * 8, np.float32), CT | OSKAR_MATRIX)
+    status = C.c_int(0)
+
+    def one():
+        lib.oskar_dftw(1, ne, k, mw.ptr, mx.ptr, my.ptr, None, 0, nd, ox.ptr,
+                       oy.ptr, None, midx.ptr, md.ptr, 1, 1, 0, out.ptr,
+                       C.byref(status))
+
+    one()
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    timer = lib.oskar_timer_create(1)
+    lib.oskar_timer_start(timer)
+    for _ in range(args.dftw_steps):
+        one()
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    t = lib.oskar_timer_elapsed(timer) / args.dftw_steps
+    lib.oskar_timer_free(timer)
+    check_status(status, "oskar_dftw")
+    rate = ne * nd / t
+    bound = 256 * 4 * 64 * 2.4e9 / 40.0
+    res = {"workload": "4096 elements x 1M directions, FP32, one element "
+                       "pattern per direction, 150 MHz",
+           "ms": round(1e3 * t, 4), "terms_per_s": rate,
+           "roofline": {"bound": "valu issue (8 VALU + 2 transcendental "
+                                 "instructions per term, 4 cycles each)",
+                        "achieved": rate, "peak": bound,
+                        "unit": "terms/s", "frac": round(rate / bound, 3)}}
+    log("dftw 4096 x 1M: %.3f ms, %.3e terms/s (%.2f of the issue bound)"
+        % (1e3 * t, rate, rate / bound))
+    ns = 20000
+    t0 = time.time()
+    oracle.dftw("single", 1, 1, k, wr, x_in, y_in, None, xo[:ns], yo[:ns],
+                None, np.zeros(ne, np.int32), data[:ns * 8])
+    tc = time.time() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res["cpu_baseline"] = {"terms_per_s": ne * ns / tc, "cores": cores,
+                           "kind": "port",
+                           "sample": "4096 elements x 20k directions"}
+    log("dftw CPU (oracle, %d cores): %.3e terms/s" % (cores,
+                                                        ne * ns / tc))
+    return res
 
 
 def bench_direct_smearing(args, rng):
