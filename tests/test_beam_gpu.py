@@ -75,6 +75,32 @@ def test_dftw_gpu_offsets_eval_flags_and_chunks(hip, prec):
         check(prec, got, want)
 
 
+@pytest.mark.parametrize("prec", ["f", "d"])
+@pytest.mark.parametrize("layout", ["one_type", "type_per_chunk"])
+def test_dftw_gpu_uniform_chunks(hip, prec, layout):
+    """1031 elements: four full LDS chunks and a 7-element tail (the
+    unrolled loop's remainder), with one data row for everything, or a
+    different data row in each chunk (a flush at every chunk boundary)."""
+    rng = np.random.default_rng(4)
+    nin, nout = 1031, 500
+    k = 2 * np.pi * 300e6 / C0
+    xi, yi = (rng.uniform(-40, 40, nin) for _ in range(2))
+    ph = rng.uniform(0, 2 * np.pi, nin)
+    w = np.stack([np.cos(ph), np.sin(ph)], 1).ravel()
+    th = rng.uniform(0, 1.5, nout)
+    phi = rng.uniform(0, 2 * np.pi, nout)
+    xo, yo = np.sin(th) * np.cos(phi), np.sin(th) * np.sin(phi)
+    if layout == "one_type":
+        idx = np.zeros(nin, np.int32)
+    else:
+        idx = (np.arange(nin) // 256).astype(np.int32)
+    data = rng.normal(size=(int(idx.max()) + 1) * nout * 8)
+    got = gpu_dftw(prec, 1, 1, k, w, xi, yi, None, xo, yo, None, idx, data)
+    want = oracle.dftw("double" if prec == "d" else "single", 1, 1, k, w,
+                       xi, yi, None, xo, yo, None, idx, data)
+    check(prec, got, want)
+
+
 def make_station(prec, ne, rng, radius=19.0, kinds=(0,), types=None,
                  is_3d=False, cable=None):
     P = OSKAR_DOUBLE if prec == "d" else OSKAR_SINGLE
