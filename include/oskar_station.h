@@ -77,6 +77,35 @@ OSKAR_EXPORT
 void oskar_station_set_element_coords(oskar_Station* station, int feed,
         int index, const double measured_enu[3], const double true_enu[3],
         int* status);
+/* oskar/telescope/station/src/oskar_station_set_element_errors.c:16-80:
+ * element gain, gain error scale, phase offset and phase error scale (the
+ * phases in degrees).  Feed 0 only (OSKAR_ERR_FUNCTION_NOT_AVAILABLE for
+ * feed 1, whose elements share feed 0's values here).  Arrays start at gain
+ * 1 and zero phase and errors.  After oskar_station_analyse, the beam
+ * multiplies each element's DFT weight by gain (cos phase, sin phase); a
+ * non-zero error scale (the reference's time-variable random errors, drawn
+ * from its counter-based generator) makes the beam evaluation return
+ * OSKAR_ERR_FUNCTION_NOT_AVAILABLE. */
+OSKAR_EXPORT
+void oskar_station_set_element_errors(oskar_Station* station, int feed,
+        int index, double gain, double gain_error, double phase_offset_deg,
+        double phase_error_deg, int* status);
+
+/* oskar/telescope/station/src/oskar_station_set_element_weight.c:13-28:
+ * complex apodisation weight of one element (feed 0; default 1 + 0i),
+ * multiplied into its DFT weight after the errors. */
+OSKAR_EXPORT
+void oskar_station_set_element_weight(oskar_Station* station, int feed,
+        int index, double re, double im, int* status);
+
+/* oskar/telescope/station/src/oskar_station_accessors.c:211-219: set by
+ * oskar_station_analyse (any gain != 1 or phase offset != 0 or error scale
+ * != 0; any apodisation weight != 1 + 0i). */
+OSKAR_EXPORT int oskar_station_apply_element_errors(
+        const oskar_Station* station);
+OSKAR_EXPORT int oskar_station_apply_element_weight(
+        const oskar_Station* station);
+
 /* oskar/telescope/station/oskar_station_set_element_cable_length_error.h */
 OSKAR_EXPORT
 void oskar_station_set_element_cable_length_error(oskar_Station* station,

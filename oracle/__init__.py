@@ -41,7 +41,7 @@ def lib():
         _lib.oracle_dipole_pattern_d.argtypes = [I, I, P, P, P, D, D, I, I, P]
         _lib.oracle_station_beam_aa_d.argtypes = (
             [I, I] + [P] * 7 + [P, I, P, P, D, D, I, I, I, I, I, D, D, D, I,
-                                P, P, P, D, P])
+                                P, P, P, D, P, P, P, P])
         _lib.oracle_station_beam_aa_d.restype = I
         _lib.oracle_evaluate_jones_K_d.argtypes = (
             [I, P, P, P, I, P, P, P, D, P, D, D, P])
@@ -212,7 +212,11 @@ def station_beam_aa(matrix, x_true, y_true, z_true, x_meas, y_meas, z_meas,
                     cable, element_types, type_kind, dipole_length_m,
                     beam_dir, x, y, z, frequency_hz, euler_x0=0.0,
                     euler_y0=0.0, swap_xy=0, is_3d=0, norm_array=0,
-                    norm_element=0, enable_array=1):
+                    norm_element=0, enable_array=1, gain=None,
+                    phase_rad=None, apod=None):
+    """Element errors (gain, phase_rad: per element, both or neither) and
+    apodisation (apod: complex per element as interleaved reals) multiply
+    the DFT weights when given."""
     f8 = np.float64
     ne = x_true.size
     npnt = x.size
@@ -228,7 +232,8 @@ def station_beam_aa(matrix, x_true, y_true, z_true, x_meas, y_meas, z_meas,
         _p(dl), float(euler_x0), float(euler_y0), int(swap_xy), int(is_3d),
         int(norm_array), int(norm_element), int(enable_array),
         float(beam_dir[0]), float(beam_dir[1]), float(beam_dir[2]), npnt,
-        *[_p(v) for v in xs], float(frequency_hz), _p(beam))
+        *[_p(v) for v in xs], float(frequency_hz),
+        *[_p(_c(v, f8)) for v in (gain, phase_rad, apod)], _p(beam))
     if r != 0:
         raise RuntimeError("oracle_station_beam_aa_d returned %d" % r)
     return beam

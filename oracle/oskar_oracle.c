@@ -473,7 +473,8 @@ int oracle_station_beam_aa_d(int matrix, int num_elements,
         int norm_array, int norm_element, int enable_array,
         double beam_x, double beam_y, double beam_z,
         int num_points, const double* x, const double* y, const double* z,
-        double frequency_hz, double* beam)
+        double frequency_hz, const double* gain, const double* phase,
+        const double* apod, double* beam)
 {
     const int nr = matrix ? 8 : 2;
     const double wavenumber = 2.0 * M_PI * frequency_hz / C_0;
@@ -519,6 +520,29 @@ int oracle_station_beam_aa_d(int matrix, int num_elements,
     {
         oracle_element_weights_dft_d(num_elements, x_meas, y_meas, z_meas,
                 cable, wavenumber, beam_x, beam_y, beam_z, w);
+        /* Element errors with zero error draws, then apodisation
+         * (oskar_station_evaluate_element_weights.c:34-64,
+         * define_evaluate_element_weights_errors.h:8-12): complex products
+         * in oskar_mem_multiply's term order. */
+        for (i = 0; i < num_elements; ++i)
+        {
+            double a = w[2 * i], b = w[2 * i + 1];
+            if (gain)
+            {
+                const double ex = cos(phase[i]) * gain[i];
+                const double ey = sin(phase[i]) * gain[i];
+                const double a2 = a * ex - b * ey, b2 = a * ey + b * ex;
+                a = a2; b = b2;
+            }
+            if (apod)
+            {
+                const double ax = apod[2 * i], ay = apod[2 * i + 1];
+                const double a2 = a * ax - b * ay, b2 = a * ay + b * ax;
+                a = a2; b = b2;
+            }
+            w[2 * i] = a;
+            w[2 * i + 1] = b;
+        }
         oracle_dftw_d(matrix, norm_array, num_elements, wavenumber, w,
                 x_true, y_true, is_3d ? z_true : 0, 0, num_points, x, y,
                 is_3d ? z : 0, element_types, signal, 1, 1, 0, beam);

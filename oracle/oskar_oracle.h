@@ -131,7 +131,8 @@ int oracle_station_beam_aa_d(int matrix, int num_elements,
         int norm_array, int norm_element, int enable_array,
         double beam_x, double beam_y, double beam_z,
         int num_points, const double* x, const double* y, const double* z,
-        double frequency_hz, double* beam);
+        double frequency_hz, const double* gain, const double* phase,
+        const double* apod, double* beam);
 
 /* Jones K: K[a][s] = exp(+i k (u_a l_s + v_a m_s + w_a (n_s - 1)))
  * (oskar/interferometer/define_evaluate_jones_K.h:22-75), where source s is

@@ -53,6 +53,38 @@ __global__ void weights_kernel(int n, const FP* x, const FP* y, const FP* z,
     }
 }
 
+/* Element errors and apodisation on the DFT weights
+ * (oskar_station_evaluate_element_weights.c:34-64): the error factor
+ * e = g (cos p, sin p) (define_evaluate_element_weights_errors.h:8-12 with
+ * zero error draws), then weights *= e and weights *= apodisation, each a
+ * complex product (a.x b.x - a.y b.y, a.x b.y + a.y b.x) with every product
+ * and sum rounded. */
+template <typename FP>
+__global__ void weights_scale_kernel(int n, Cplx<FP>* w, const FP* gain,
+        const FP* phase, const Cplx<FP>* apod)
+{
+#pragma clang fp contract(off)
+    GRID_STRIDE(i, (size_t) n)
+    {
+        Cplx<FP> x = w[i];
+        if (gain)
+        {
+            FP s, c;
+            sincos_v<FP>(phase[i], &s, &c);
+            const FP ex = c * gain[i], ey = s * gain[i];
+            const Cplx<FP> y = {x.x * ex - x.y * ey, x.x * ey + x.y * ex};
+            x = y;
+        }
+        if (apod)
+        {
+            const Cplx<FP> a = apod[i];
+            const Cplx<FP> y = {x.x * a.x - x.y * a.y, x.x * a.y + x.y * a.x};
+            x = y;
+        }
+        w[i] = x;
+    }
+}
+
 template <typename FP>
 __global__ void theta_phi_kernel(int off_in, int num, const FP* x,
         const FP* y, const FP* z, int extra_point_at_pole, FP delta_phi1,
@@ -303,6 +335,25 @@ void join_vec_kernel(size_t n, JonesMat<FP>* out, const void* a, const void* b)
 }
 
 } /* namespace */
+
+extern "C" int oskar_amd_weights_scale_launch(int precision, int num,
+        void* weights, const void* gain, const void* phase, const void* apod,
+        void* stream)
+{
+    if (num <= 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+    if (precision == OSKAR_DOUBLE)
+        hipLaunchKernelGGL(weights_scale_kernel<double>, grid_for(num),
+                dim3(THREADS), 0, s, num, (Cplx<double>*) weights,
+                (const double*) gain, (const double*) phase,
+                (const Cplx<double>*) apod);
+    else
+        hipLaunchKernelGGL(weights_scale_kernel<float>, grid_for(num),
+                dim3(THREADS), 0, s, num, (Cplx<float>*) weights,
+                (const float*) gain, (const float*) phase,
+                (const Cplx<float>*) apod);
+    return (int) hipGetLastError();
+}
 
 extern "C" int oskar_amd_element_weights_dft_launch(int precision,
         int num_elements, const void* x, const void* y, const void* z,

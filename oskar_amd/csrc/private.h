@@ -97,6 +97,19 @@ struct oskar_Station
     oskar_Mem* element_types;     /* int, station location */
     oskar_Mem* element_types_cpu; /* int, host */
     oskar_Element** element;
+    /* Per-element gain, phase offset, the scales of their time-variable
+     * errors, and apodisation weights (private_station.h element_gain,
+     * element_gain_error, element_phase_offset_rad, element_phase_error_rad,
+     * element_weight; feed 0; NULL until first set).  The apply flags are
+     * set by oskar_station_analyse. */
+    oskar_Mem* element_gain;
+    oskar_Mem* element_gain_error;
+    oskar_Mem* element_phase_offset_rad;
+    oskar_Mem* element_phase_error_rad;
+    oskar_Mem* element_weight;        /* complex */
+    int apply_element_errors;
+    int element_errors_random;        /* a gain or phase error scale != 0 */
+    int apply_element_weight;
     int num_permitted_beams;          /* private_station.h:91-93 */
     oskar_Mem* permitted_beam_az_rad; /* double, host; NULL until loaded */
     oskar_Mem* permitted_beam_el_rad;
@@ -312,6 +325,10 @@ int oskar_amd_station_to_baseline_launch(int precision, int casa,
         int num_times, int num_stations, int offset_in, const void* u,
         const void* v, const void* w, int offset_out, void* uu, void* vv,
         void* ww, void* stream);
+/* weights[i] *= gain[i] (cos phase[i], sin phase[i]) if gain, then
+ * *= apod[i] if apod (complex multiplies in the reference's term order). */
+int oskar_amd_weights_scale_launch(int precision, int num, void* weights,
+        const void* gain, const void* phase, const void* apod, void* stream);
 int oskar_amd_enu_convert_launch(int precision, int kind, int num,
         const void* a, const void* b, const void* c, const double* params,
         void* x, void* y, void* z, void* stream);

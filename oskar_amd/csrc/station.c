@@ -148,6 +148,11 @@ void oskar_station_free(oskar_Station* s, int* status)
     oskar_mem_free(s->element_types_cpu, status);
     oskar_mem_free(s->permitted_beam_az_rad, status);
     oskar_mem_free(s->permitted_beam_el_rad, status);
+    oskar_mem_free(s->element_gain, status);
+    oskar_mem_free(s->element_gain_error, status);
+    oskar_mem_free(s->element_phase_offset_rad, status);
+    oskar_mem_free(s->element_phase_error_rad, status);
+    oskar_mem_free(s->element_weight, status);
     for (i = 0; i < s->num_element_types; ++i) free(s->element[i]);
     free(s->element);
     free(s);
@@ -228,6 +233,20 @@ oskar_Station* oskar_station_create_copy(const oskar_Station* src,
     oskar_station_resize_element_types(s, src->num_element_types, status);
     for (i = 0; i < src->num_element_types && !*status; ++i)
         *s->element[i] = *src->element[i];
+    {
+        oskar_Mem** d[5] = {&s->element_gain, &s->element_gain_error,
+                &s->element_phase_offset_rad, &s->element_phase_error_rad,
+                &s->element_weight};
+        const oskar_Mem* m[5] = {src->element_gain, src->element_gain_error,
+                src->element_phase_offset_rad, src->element_phase_error_rad,
+                src->element_weight};
+        for (i = 0; i < 5; ++i)
+            if (m[i] && !*status)
+                *d[i] = oskar_mem_create_copy(m[i], location, status);
+        s->apply_element_errors = src->apply_element_errors;
+        s->element_errors_random = src->element_errors_random;
+        s->apply_element_weight = src->apply_element_weight;
+    }
     if (src->num_permitted_beams > 0 && !*status)
     {
         s->permitted_beam_az_rad = oskar_mem_create_copy(
@@ -303,6 +322,78 @@ void oskar_station_set_element_cable_length_error(oskar_Station* s, int feed,
     set_real(s->element_cable_length_error, index, error_metres, status);
 }
 
+/* A per-element array of the station, created on first use with every
+ * element at the default value (real part; imaginary parts zero). */
+static oskar_Mem* element_array(oskar_Station* s, oskar_Mem** m, int type,
+        double value, int* status)
+{
+    if (!*m && !*status)
+    {
+        *m = oskar_mem_create(type, s->mem_location, (size_t) s->num_elements,
+                status);
+        oskar_mem_set_value_real(*m, value, 0, (size_t) s->num_elements,
+                status);
+    }
+    return *m;
+}
+
+void oskar_station_set_element_errors(oskar_Station* s, int feed, int index,
+        double gain, double gain_error, double phase_offset_deg,
+        double phase_error_deg, int* status)
+{
+    if (*status || !s) return;
+    if (feed != 0) { *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE; return; }
+    if (index < 0 || index >= s->num_elements)
+    {
+        *status = OSKAR_ERR_OUT_OF_RANGE;
+        return;
+    }
+    const int t = s->precision;
+    set_real(element_array(s, &s->element_gain, t, 1.0, status), index, gain,
+            status);
+    set_real(element_array(s, &s->element_gain_error, t, 0.0, status), index,
+            gain_error, status);
+    set_real(element_array(s, &s->element_phase_offset_rad, t, 0.0, status),
+            index, phase_offset_deg * M_PI / 180.0, status);
+    set_real(element_array(s, &s->element_phase_error_rad, t, 0.0, status),
+            index, phase_error_deg * M_PI / 180.0, status);
+}
+
+void oskar_station_set_element_weight(oskar_Station* s, int feed, int index,
+        double re, double im, int* status)
+{
+    if (*status || !s) return;
+    if (feed != 0) { *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE; return; }
+    if (index < 0 || index >= s->num_elements)
+    {
+        *status = OSKAR_ERR_OUT_OF_RANGE;
+        return;
+    }
+    oskar_Mem* w = element_array(s, &s->element_weight,
+            s->precision | OSKAR_COMPLEX, 1.0, status);
+    if (*status) return;
+    {
+        /* One complex value through a host alias. */
+        double vd[2] = {re, im};
+        float vf[2] = {(float) re, (float) im};
+        oskar_Mem* src = oskar_mem_create_alias_from_raw(
+                s->precision == OSKAR_DOUBLE ? (void*) vd : (void*) vf,
+                s->precision | OSKAR_COMPLEX, OSKAR_CPU, 1, status);
+        oskar_mem_copy_contents(w, src, (size_t) index, 0, 1, status);
+        oskar_mem_free(src, status);
+    }
+}
+
+int oskar_station_apply_element_errors(const oskar_Station* s)
+{
+    return s ? s->apply_element_errors : 0;
+}
+
+int oskar_station_apply_element_weight(const oskar_Station* s)
+{
+    return s ? s->apply_element_weight : 0;
+}
+
 void oskar_station_set_element_feed_angle(oskar_Station* s, int feed,
         int index, double alpha_deg, double beta_deg, double gamma_deg,
         int* status)
@@ -367,6 +458,50 @@ void oskar_station_analyse(oskar_Station* s,
             {
                 const double v = oskar_mem_get_element(h, (size_t) i, status);
                 if (v != 0.0) { s->array_is_3d = 1; break; }
+            }
+            oskar_mem_free(h, status);
+        }
+    }
+    /* Element errors and apodisation (oskar_station_analyse.c:110-133):
+     * errors apply if any gain != 1 or phase offset != 0, or any error
+     * scale != 0; apodisation if any weight != 1 + 0i. */
+    s->apply_element_errors = 0;
+    s->element_errors_random = 0;
+    s->apply_element_weight = 0;
+    {
+        const oskar_Mem* m[4] = {s->element_gain, s->element_phase_offset_rad,
+                s->element_gain_error, s->element_phase_error_rad};
+        const double dflt[4] = {1.0, 0.0, 0.0, 0.0};
+        for (d = 0; d < 4 && !*status; ++d)
+        {
+            if (!m[d]) continue;
+            oskar_Mem* h = oskar_mem_create_copy(m[d], OSKAR_CPU, status);
+            for (i = 0; i < s->num_elements && !*status; ++i)
+                if (oskar_mem_get_element(h, (size_t) i, status) != dflt[d])
+                {
+                    s->apply_element_errors = 1;
+                    if (d >= 2) s->element_errors_random = 1;
+                    break;
+                }
+            oskar_mem_free(h, status);
+        }
+        if (s->element_weight && !*status)
+        {
+            oskar_Mem* h = oskar_mem_create_copy(s->element_weight, OSKAR_CPU,
+                    status);
+            const int dbl = s->precision == OSKAR_DOUBLE;
+            const void* p = oskar_mem_void_const(h);
+            for (i = 0; i < s->num_elements && !*status; ++i)
+            {
+                const double re = dbl ? ((const double*) p)[2 * i] :
+                        ((const float*) p)[2 * i];
+                const double im = dbl ? ((const double*) p)[2 * i + 1] :
+                        ((const float*) p)[2 * i + 1];
+                if (re != 1.0 || im != 0.0)
+                {
+                    s->apply_element_weight = 1;
+                    break;
+                }
             }
             oskar_mem_free(h, status);
         }
@@ -938,6 +1073,26 @@ void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
                 s->element_measured_enu_metres[2],
                 s->element_cable_length_error, wavenumber, beam_x, beam_y,
                 beam_z, work->weights, status);
+        if ((s->apply_element_errors || s->apply_element_weight) && !*status)
+        {
+            /* Time-variable errors need the reference's random draws, which
+             * this library does not reproduce. */
+            if (s->element_errors_random)
+            {
+                *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE;
+                return;
+            }
+            const int e = oskar_amd_weights_scale_launch(s->precision,
+                    s->num_elements, oskar_mem_void(work->weights),
+                    s->apply_element_errors ?
+                    oskar_mem_void_const(s->element_gain) : 0,
+                    s->apply_element_errors ?
+                    oskar_mem_void_const(s->element_phase_offset_rad) : 0,
+                    s->apply_element_weight ?
+                    oskar_mem_void_const(s->element_weight) : 0,
+                    oskar_amd_current_stream());
+            if (e) { *status = e; return; }
+        }
         oskar_dftw(s->normalise_array_pattern, s->num_elements, wavenumber,
                 work->weights, s->element_true_enu_metres[0],
                 s->element_true_enu_metres[1], s->element_true_enu_metres[2],
