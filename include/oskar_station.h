@@ -82,10 +82,12 @@ void oskar_station_set_element_coords(oskar_Station* station, int feed,
  * phases in degrees).  Feed 0 only (OSKAR_ERR_FUNCTION_NOT_AVAILABLE for
  * feed 1, whose elements share feed 0's values here).  Arrays start at gain
  * 1 and zero phase and errors.  After oskar_station_analyse, the beam
- * multiplies each element's DFT weight by gain (cos phase, sin phase); a
- * non-zero error scale (the reference's time-variable random errors, drawn
- * from its counter-based generator) makes the beam evaluation return
- * OSKAR_ERR_FUNCTION_NOT_AVAILABLE. */
+ * multiplies each element's DFT weight by a (cos p, sin p) with
+ * a = gain + g gain_error and p = phase_offset + h phase_error, where
+ * (g, h) are the element's Gaussian draws of the reference's generator
+ * (Philox4x32-10 keyed by the station's seed, counter (element / 2,
+ * time index, station unique id, 0x12345678), Box-Muller), zero when no
+ * error scale is set. */
 OSKAR_EXPORT
 void oskar_station_set_element_errors(oskar_Station* station, int feed,
         int index, double gain, double gain_error, double phase_offset_deg,
@@ -97,6 +99,18 @@ void oskar_station_set_element_errors(oskar_Station* station, int feed,
 OSKAR_EXPORT
 void oskar_station_set_element_weight(oskar_Station* station, int feed,
         int index, double re, double im, int* status);
+
+/* oskar/telescope/station/src/oskar_station_accessors.c:21-24, 221-224,
+ * 503-513, 615-620: the seed of the time-variable element errors (default
+ * 1) and the station's unique id (the next value of *counter), both part of
+ * the random draws' key and counter. */
+OSKAR_EXPORT unsigned int oskar_station_seed_time_variable_errors(
+        const oskar_Station* station);
+OSKAR_EXPORT void oskar_station_set_seed_time_variable_errors(
+        oskar_Station* station, unsigned int value);
+OSKAR_EXPORT int oskar_station_unique_id(const oskar_Station* station);
+OSKAR_EXPORT void oskar_station_set_unique_ids(oskar_Station* station,
+        int* counter);
 
 /* oskar/telescope/station/src/oskar_station_accessors.c:211-219: set by
  * oskar_station_analyse (any gain != 1 or phase offset != 0 or error scale

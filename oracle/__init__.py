@@ -41,7 +41,7 @@ def lib():
         _lib.oracle_dipole_pattern_d.argtypes = [I, I, P, P, P, D, D, I, I, P]
         _lib.oracle_station_beam_aa_d.argtypes = (
             [I, I] + [P] * 7 + [P, I, P, P, D, D, I, I, I, I, I, D, D, D, I,
-                                P, P, P, D, P, P, P, P])
+                                P, P, P, D, P, P, P])
         _lib.oracle_station_beam_aa_d.restype = I
         _lib.oracle_evaluate_jones_K_d.argtypes = (
             [I, P, P, P, I, P, P, P, D, P, D, D, P])
@@ -71,6 +71,10 @@ def lib():
         DP = C.POINTER(C.c_double)
         _lib.oracle_snap_beam_direction.argtypes = [DP, DP, DP, I, P, P]
         _lib.oracle_snap_beam_direction.restype = I
+        U = C.c_uint
+        _lib.oracle_philox4x32_10.argtypes = [P, P, P]
+        _lib.oracle_random_gaussian_d.argtypes = [U, U, U, U, U, P]
+        _lib.oracle_element_errors_d.argtypes = [I, U, I, I, P, P, P, P, P]
     return _lib
 
 
@@ -212,11 +216,10 @@ def station_beam_aa(matrix, x_true, y_true, z_true, x_meas, y_meas, z_meas,
                     cable, element_types, type_kind, dipole_length_m,
                     beam_dir, x, y, z, frequency_hz, euler_x0=0.0,
                     euler_y0=0.0, swap_xy=0, is_3d=0, norm_array=0,
-                    norm_element=0, enable_array=1, gain=None,
-                    phase_rad=None, apod=None):
-    """Element errors (gain, phase_rad: per element, both or neither) and
-    apodisation (apod: complex per element as interleaved reals) multiply
-    the DFT weights when given."""
+                    norm_element=0, enable_array=1, err=None, apod=None):
+    """Element error factors (err, from element_errors) and apodisation
+    (apod), complex per element as interleaved reals, multiply the DFT
+    weights when given."""
     f8 = np.float64
     ne = x_true.size
     npnt = x.size
@@ -233,7 +236,7 @@ def station_beam_aa(matrix, x_true, y_true, z_true, x_meas, y_meas, z_meas,
         int(norm_array), int(norm_element), int(enable_array),
         float(beam_dir[0]), float(beam_dir[1]), float(beam_dir[2]), npnt,
         *[_p(v) for v in xs], float(frequency_hz),
-        *[_p(_c(v, f8)) for v in (gain, phase_rad, apod)], _p(beam))
+        *[_p(_c(v, f8)) for v in (err, apod)], _p(beam))
     if r != 0:
         raise RuntimeError("oracle_station_beam_aa_d returned %d" % r)
     return beam
@@ -397,3 +400,28 @@ def snap_beam_direction(xyz, az, el):
     k = lib().oracle_snap_beam_direction(C.byref(x), C.byref(y), C.byref(z),
                                          az.size, _p(az), _p(el))
     return (x.value, y.value, z.value), k
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().oracle_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def random_gaussian(n, seed, c1, c2, c3):
+    out = np.zeros(n)
+    lib().oracle_random_gaussian_d(n, seed, c1, c2, c3, _p(out))
+    return out
+
+
+def element_errors(seed, time_index, station_id, gain, gain_error, phase,
+                   phase_error):
+    """Interleaved complex error factors of the elements."""
+    f8 = np.float64
+    arr = [_c(v, f8) for v in (gain, gain_error, phase, phase_error)]
+    err = np.zeros(2 * arr[0].size)
+    lib().oracle_element_errors_d(arr[0].size, seed, time_index, station_id,
+                                  *[_p(v) for v in arr], _p(err))
+    return err

@@ -473,8 +473,8 @@ int oracle_station_beam_aa_d(int matrix, int num_elements,
         int norm_array, int norm_element, int enable_array,
         double beam_x, double beam_y, double beam_z,
         int num_points, const double* x, const double* y, const double* z,
-        double frequency_hz, const double* gain, const double* phase,
-        const double* apod, double* beam)
+        double frequency_hz, const double* err, const double* apod,
+        double* beam)
 {
     const int nr = matrix ? 8 : 2;
     const double wavenumber = 2.0 * M_PI * frequency_hz / C_0;
@@ -520,17 +520,15 @@ int oracle_station_beam_aa_d(int matrix, int num_elements,
     {
         oracle_element_weights_dft_d(num_elements, x_meas, y_meas, z_meas,
                 cable, wavenumber, beam_x, beam_y, beam_z, w);
-        /* Element errors with zero error draws, then apodisation
-         * (oskar_station_evaluate_element_weights.c:34-64,
-         * define_evaluate_element_weights_errors.h:8-12): complex products
-         * in oskar_mem_multiply's term order. */
+        /* Element error factors (oracle_element_errors_d), then
+         * apodisation (oskar_station_evaluate_element_weights.c:34-64):
+         * complex products in oskar_mem_multiply's term order. */
         for (i = 0; i < num_elements; ++i)
         {
             double a = w[2 * i], b = w[2 * i + 1];
-            if (gain)
+            if (err)
             {
-                const double ex = cos(phase[i]) * gain[i];
-                const double ey = sin(phase[i]) * gain[i];
+                const double ex = err[2 * i], ey = err[2 * i + 1];
                 const double a2 = a * ex - b * ey, b2 = a * ey + b * ex;
                 a = a2; b = b2;
             }
@@ -1056,4 +1054,103 @@ int oracle_snap_beam_direction(double* x, double* y, double* z, int n,
         *z = sin(p_el[best]);
     }
     return best;
+}
+
+/*
+ * Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers:
+ * as easy as 1, 2, 3", SC 2011), the counter-based generator the reference
+ * takes from Random123 (oskar/math/private_random_generators.h): ten rounds
+ * of two 32x32 -> 64-bit products with multipliers 0xD2511F53 and
+ * 0xCD9E8D57, the key bumped by 0x9E3779B9 and 0xBB67AE85 between rounds.
+ */
+void oracle_philox4x32_10(const unsigned int* ctr, const unsigned int* key,
+        unsigned int* out)
+{
+    unsigned int c[4], k0 = key[0], k1 = key[1];
+    int r, i;
+    for (i = 0; i < 4; ++i) c[i] = ctr[i];
+    for (r = 0; r < 10; ++r)
+    {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        {
+            const unsigned long long p0 = 0xD2511F53ull * c[0];
+            const unsigned long long p1 = 0xCD9E8D57ull * c[2];
+            const unsigned int n0 = (unsigned int) (p1 >> 32) ^ c[1] ^ k0;
+            const unsigned int n2 = (unsigned int) (p0 >> 32) ^ c[3] ^ k1;
+            c[0] = n0;
+            c[1] = (unsigned int) p1;
+            c[2] = n2;
+            c[3] = (unsigned int) p0;
+        }
+    }
+    for (i = 0; i < 4; ++i) out[i] = c[i];
+}
+
+/* Box-Muller of two 32-bit integers, the reference's CPU form
+ * (oskar/math/private_random_helpers.h): the angle pi x u0 / 2^31 + half
+ * step (u0 read as a 64-bit long, so the angle spans [0, 2 pi)), the radius
+ * sqrt(-2 ln((u1 + 1/2) / 2^32)). */
+static void box_muller_d(unsigned int u0, unsigned int u1, double* f0,
+        double* f1)
+{
+    const double fa = 1.0 / (1.0 + 0x7FFFFFFFuL);
+    const double fb = 1.0 / (1.0 + 0xFFFFFFFFuL);
+    double t = M_PI;
+    t *= (((long) (unsigned long) u0) * fa) + 0.5 * fa;
+    *f0 = sin(t);
+    *f1 = cos(t);
+    {
+        const double r = sqrt(-2.0 * log((u1 * fb) + 0.5 * fb));
+        *f0 *= r;
+        *f1 *= r;
+    }
+}
+
+/* Gaussian draws of oskar_mem_random_gaussian (oskar/mem/src/
+ * oskar_mem_random_gaussian.c:90-129, double, std 1): value j comes from
+ * Philox with key (seed, 0xCAFEF00D) and counter (j / 4, c1, c2, c3),
+ * Box-Muller of its outputs 0-1 for j % 4 in {0, 1} and 2-3 for {2, 3}. */
+void oracle_random_gaussian_d(unsigned int n, unsigned int seed,
+        unsigned int c1, unsigned int c2, unsigned int c3, double* out)
+{
+    unsigned int i;
+    for (i = 0; i < (n + 3) / 4; ++i)
+    {
+        const unsigned int ctr[4] = {i, c1, c2, c3};
+        const unsigned int key[2] = {seed, 0xCAFEF00Du};
+        unsigned int u[4];
+        double t[4];
+        unsigned int j;
+        oracle_philox4x32_10(ctr, key, u);
+        box_muller_d(u[0], u[1], &t[0], &t[1]);
+        box_muller_d(u[2], u[3], &t[2], &t[3]);
+        for (j = 0; j < 4 && 4 * i + j < n; ++j) out[4 * i + j] = t[j];
+    }
+}
+
+/* Complex error factors of the station's elements
+ * (oskar/telescope/station/src/oskar_evaluate_element_weights_errors.c,
+ * define_evaluate_element_weights_errors.h:8-12): draws (r.x, r.y) per
+ * element from oracle_random_gaussian_d(2 n, seed, time_index, station_id,
+ * 0x12345678), amplitude r.x gain_error + gain, phase r.y phase_error +
+ * phase, factor amplitude (cos, sin)(phase).  Output interleaved. */
+void oracle_element_errors_d(int n, unsigned int seed, int time_index,
+        int station_id, const double* gain, const double* gain_error,
+        const double* phase, const double* phase_error, double* err)
+{
+    int i;
+    oracle_random_gaussian_d((unsigned int) (2 * n), seed,
+            (unsigned int) time_index, (unsigned int) station_id,
+            0x12345678u, err);
+    for (i = 0; i < n; ++i)
+    {
+        double rx = err[2 * i], ry = err[2 * i + 1];
+        rx *= gain_error[i]; rx += gain[i];
+        ry *= phase_error[i]; ry += phase[i];
+        {
+            const double im = sin(ry), re = cos(ry);
+            err[2 * i] = re * rx;
+            err[2 * i + 1] = im * rx;
+        }
+    }
 }

@@ -131,8 +131,8 @@ int oracle_station_beam_aa_d(int matrix, int num_elements,
         int norm_array, int norm_element, int enable_array,
         double beam_x, double beam_y, double beam_z,
         int num_points, const double* x, const double* y, const double* z,
-        double frequency_hz, const double* gain, const double* phase,
-        const double* apod, double* beam);
+        double frequency_hz, const double* err, const double* apod,
+        double* beam);
 
 /* Jones K: K[a][s] = exp(+i k (u_a l_s + v_a m_s + w_a (n_s - 1)))
  * (oskar/interferometer/define_evaluate_jones_K.h:22-75), where source s is
@@ -184,6 +184,17 @@ double oracle_angular_distance(double lon1_rad, double lon2_rad,
  * (oskar_station_beam_horizon_direction.c:50-88); returns its index. */
 int oracle_snap_beam_direction(double* x, double* y, double* z, int n,
         const double* p_az, const double* p_el);
+
+/* Philox4x32-10 (the reference's Random123 generator). */
+void oracle_philox4x32_10(const unsigned int* ctr, const unsigned int* key,
+        unsigned int* out);
+/* oskar_mem_random_gaussian's draws (double, std 1). */
+void oracle_random_gaussian_d(unsigned int n, unsigned int seed,
+        unsigned int c1, unsigned int c2, unsigned int c3, double* out);
+/* Complex error factors of a station's elements for one time index. */
+void oracle_element_errors_d(int n, unsigned int seed, int time_index,
+        int station_id, const double* gain, const double* gain_error,
+        const double* phase, const double* phase_error, double* err);
 
 #ifdef __cplusplus
 }

@@ -53,31 +53,95 @@ __global__ void weights_kernel(int n, const FP* x, const FP* y, const FP* z,
     }
 }
 
-/* Element errors and apodisation on the DFT weights
- * (oskar_station_evaluate_element_weights.c:34-64): the error factor
- * e = g (cos p, sin p) (define_evaluate_element_weights_errors.h:8-12 with
- * zero error draws), then weights *= e and weights *= apodisation, each a
- * complex product (a.x b.x - a.y b.y, a.x b.y + a.y b.x) with every product
- * and sum rounded. */
+/* Philox4x32-10 (Salmon et al., SC 2011), the reference's Random123
+ * generator (oskar/math/private_random_generators.h). */
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, unsigned int k0,
+        unsigned int k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r)
+    {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const unsigned int hi0 = __umulhi(0xD2511F53u, c.x);
+        const unsigned int lo0 = 0xD2511F53u * c.x;
+        const unsigned int hi1 = __umulhi(0xCD9E8D57u, c.z);
+        const unsigned int lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    }
+    return c;
+}
+
+/* Box-Muller of two 32-bit integers in the reference's CPU form
+ * (oskar/math/private_random_helpers.h): angle pi (u0 / 2^31 + half step)
+ * (u0 read as a 64-bit long), radius sqrt(-2 ln((u1 + 1/2) / 2^32)), in the
+ * precision of FP. */
 template <typename FP>
-__global__ void weights_scale_kernel(int n, Cplx<FP>* w, const FP* gain,
-        const FP* phase, const Cplx<FP>* apod)
+__device__ __forceinline__ void box_muller(unsigned int u0, unsigned int u1,
+        FP* f0, FP* f1)
+{
+#pragma clang fp contract(off)
+    const FP fa = (FP) 1 / ((FP) 1 + (FP) 0x7FFFFFFFu);
+    const FP fb = (FP) 1 / ((FP) 1 + (FP) 0xFFFFFFFFu);
+    FP t = (FP) 3.14159265358979323846;
+    t *= ((FP) (long long) u0 * fa) + (FP) 0.5 * fa;
+    FP s, c;
+    sincos_v<FP>(t, &s, &c);
+    const FP r = sqrt((FP) -2 * log(((FP) u1 * fb) + (FP) 0.5 * fb));
+    *f0 = s * r;
+    *f1 = c * r;
+}
+
+/* Element errors and apodisation on the DFT weights
+ * (oskar_station_evaluate_element_weights.c:34-64):
+ *  - error draws (r.x, r.y) for element i are values 2i and 2i+1 of
+ *    oskar_mem_random_gaussian(seed, time_index, station_id, 0x12345678)
+ *    (Philox counter i / 2, its outputs 0-1 or 2-3), or zero when no error
+ *    scale is set;
+ *  - amplitude r.x gain_error + gain, phase r.y phase_error + phase, factor
+ *    amplitude (cos, sin)(phase) (define_evaluate_element_weights_errors.h
+ *    :8-12);
+ *  - weights *= factor, then *= apodisation, each a complex product
+ *    (a.x b.x - a.y b.y, a.x b.y + a.y b.x) with every product and sum
+ *    rounded. */
+template <typename FP>
+struct ErrArgs
+{
+    const FP *gain, *gain_error, *phase, *phase_error;
+    const Cplx<FP>* apod;
+    unsigned int seed, time_index, station_id;
+};
+
+template <typename FP>
+__global__ void weights_scale_kernel(int n, Cplx<FP>* w, ErrArgs<FP> e)
 {
 #pragma clang fp contract(off)
     GRID_STRIDE(i, (size_t) n)
     {
         Cplx<FP> x = w[i];
-        if (gain)
+        if (e.gain)
         {
+            FP rx = (FP) 0, ry = (FP) 0;
+            if (e.gain_error)
+            {
+                const uint4 u = philox4x32_10(make_uint4((unsigned int) i / 2,
+                        e.time_index, e.station_id, 0x12345678u), e.seed,
+                        0xCAFEF00Du);
+                if (i % 2 == 0) box_muller<FP>(u.x, u.y, &rx, &ry);
+                else box_muller<FP>(u.z, u.w, &rx, &ry);
+                rx *= e.gain_error[i];
+                ry *= e.phase_error[i];
+            }
+            rx += e.gain[i];
+            ry += e.phase[i];
             FP s, c;
-            sincos_v<FP>(phase[i], &s, &c);
-            const FP ex = c * gain[i], ey = s * gain[i];
+            sincos_v<FP>(ry, &s, &c);
+            const FP ex = c * rx, ey = s * rx;
             const Cplx<FP> y = {x.x * ex - x.y * ey, x.x * ey + x.y * ex};
             x = y;
         }
-        if (apod)
+        if (e.apod)
         {
-            const Cplx<FP> a = apod[i];
+            const Cplx<FP> a = e.apod[i];
             const Cplx<FP> y = {x.x * a.x - x.y * a.y, x.x * a.y + x.y * a.x};
             x = y;
         }
@@ -337,21 +401,30 @@ void join_vec_kernel(size_t n, JonesMat<FP>* out, const void* a, const void* b)
 } /* namespace */
 
 extern "C" int oskar_amd_weights_scale_launch(int precision, int num,
-        void* weights, const void* gain, const void* phase, const void* apod,
-        void* stream)
+        void* weights, const void* gain, const void* gain_error,
+        const void* phase, const void* phase_error, const void* apod,
+        unsigned int seed, int time_index, int station_id, void* stream)
 {
     if (num <= 0) return 0;
     const hipStream_t s = (hipStream_t) stream;
     if (precision == OSKAR_DOUBLE)
+    {
+        const ErrArgs<double> e = {(const double*) gain,
+                (const double*) gain_error, (const double*) phase,
+                (const double*) phase_error, (const Cplx<double>*) apod, seed,
+                (unsigned int) time_index, (unsigned int) station_id};
         hipLaunchKernelGGL(weights_scale_kernel<double>, grid_for(num),
-                dim3(THREADS), 0, s, num, (Cplx<double>*) weights,
-                (const double*) gain, (const double*) phase,
-                (const Cplx<double>*) apod);
+                dim3(THREADS), 0, s, num, (Cplx<double>*) weights, e);
+    }
     else
+    {
+        const ErrArgs<float> e = {(const float*) gain,
+                (const float*) gain_error, (const float*) phase,
+                (const float*) phase_error, (const Cplx<float>*) apod, seed,
+                (unsigned int) time_index, (unsigned int) station_id};
         hipLaunchKernelGGL(weights_scale_kernel<float>, grid_for(num),
-                dim3(THREADS), 0, s, num, (Cplx<float>*) weights,
-                (const float*) gain, (const float*) phase,
-                (const Cplx<float>*) apod);
+                dim3(THREADS), 0, s, num, (Cplx<float>*) weights, e);
+    }
     return (int) hipGetLastError();
 }
 

@@ -109,6 +109,8 @@ struct oskar_Station
     oskar_Mem* element_weight;        /* complex */
     int apply_element_errors;
     int element_errors_random;        /* a gain or phase error scale != 0 */
+    unsigned int seed_time_variable_errors;   /* private_station.h; 1 */
+    int unique_id;
     int apply_element_weight;
     int num_permitted_beams;          /* private_station.h:91-93 */
     oskar_Mem* permitted_beam_az_rad; /* double, host; NULL until loaded */
@@ -325,10 +327,14 @@ int oskar_amd_station_to_baseline_launch(int precision, int casa,
         int num_times, int num_stations, int offset_in, const void* u,
         const void* v, const void* w, int offset_out, void* uu, void* vv,
         void* ww, void* stream);
-/* weights[i] *= gain[i] (cos phase[i], sin phase[i]) if gain, then
- * *= apod[i] if apod (complex multiplies in the reference's term order). */
+/* weights[i] *= amplitude (cos, sin)(phase) if gain (amplitude = gain +
+ * draw x gain_error, phase = phase + draw x phase_error, the Gaussian draws
+ * of the reference's generator for (seed, time_index, station_id) when
+ * gain_error is set, else none), then *= apod[i] if apod. */
 int oskar_amd_weights_scale_launch(int precision, int num, void* weights,
-        const void* gain, const void* phase, const void* apod, void* stream);
+        const void* gain, const void* gain_error, const void* phase,
+        const void* phase_error, const void* apod, unsigned int seed,
+        int time_index, int station_id, void* stream);
 int oskar_amd_enu_convert_launch(int precision, int kind, int num,
         const void* a, const void* b, const void* c, const double* params,
         void* x, void* y, void* z, void* stream);

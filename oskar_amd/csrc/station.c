@@ -103,6 +103,7 @@ oskar_Station* oskar_station_create(int type, int location, int num_elements,
     s->enable_array_pattern = OSKAR_TRUE;
     s->common_element_orientation = OSKAR_TRUE;
     s->common_pol_beams = OSKAR_TRUE;
+    s->seed_time_variable_errors = 1;   /* oskar_station_create.c:100 */
     for (i = 0; i < 3; ++i)
     {
         s->element_true_enu_metres[i] =
@@ -244,6 +245,8 @@ oskar_Station* oskar_station_create_copy(const oskar_Station* src,
             if (m[i] && !*status)
                 *d[i] = oskar_mem_create_copy(m[i], location, status);
         s->apply_element_errors = src->apply_element_errors;
+        s->seed_time_variable_errors = src->seed_time_variable_errors;
+        s->unique_id = src->unique_id;
         s->element_errors_random = src->element_errors_random;
         s->apply_element_weight = src->apply_element_weight;
     }
@@ -382,6 +385,27 @@ void oskar_station_set_element_weight(oskar_Station* s, int feed, int index,
         oskar_mem_copy_contents(w, src, (size_t) index, 0, 1, status);
         oskar_mem_free(src, status);
     }
+}
+
+unsigned int oskar_station_seed_time_variable_errors(const oskar_Station* s)
+{
+    return s ? s->seed_time_variable_errors : 0u;
+}
+
+void oskar_station_set_seed_time_variable_errors(oskar_Station* s,
+        unsigned int value)
+{
+    if (s) s->seed_time_variable_errors = value;
+}
+
+int oskar_station_unique_id(const oskar_Station* s)
+{
+    return s ? s->unique_id : 0;
+}
+
+void oskar_station_set_unique_ids(oskar_Station* s, int* counter)
+{
+    if (s && counter) s->unique_id = (*counter)++;
 }
 
 int oskar_station_apply_element_errors(const oskar_Station* s)
@@ -1017,7 +1041,6 @@ void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
 {
     double beam_x = 0.0, beam_y = 0.0, beam_z = 0.0;
     int i = 0;
-    (void) time_index;
     if (*status) return;
     if (s->station_type != OSKAR_STATION_TYPE_AA ||
             !s->common_element_orientation || !s->common_pol_beams)
@@ -1075,21 +1098,17 @@ void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
                 beam_z, work->weights, status);
         if ((s->apply_element_errors || s->apply_element_weight) && !*status)
         {
-            /* Time-variable errors need the reference's random draws, which
-             * this library does not reproduce. */
-            if (s->element_errors_random)
-            {
-                *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE;
-                return;
-            }
+            const int err = s->apply_element_errors;
+            const int rnd = err && s->element_errors_random;
             const int e = oskar_amd_weights_scale_launch(s->precision,
                     s->num_elements, oskar_mem_void(work->weights),
-                    s->apply_element_errors ?
-                    oskar_mem_void_const(s->element_gain) : 0,
-                    s->apply_element_errors ?
-                    oskar_mem_void_const(s->element_phase_offset_rad) : 0,
+                    err ? oskar_mem_void_const(s->element_gain) : 0,
+                    rnd ? oskar_mem_void_const(s->element_gain_error) : 0,
+                    err ? oskar_mem_void_const(s->element_phase_offset_rad) : 0,
+                    rnd ? oskar_mem_void_const(s->element_phase_error_rad) : 0,
                     s->apply_element_weight ?
                     oskar_mem_void_const(s->element_weight) : 0,
+                    s->seed_time_variable_errors, time_index, s->unique_id,
                     oskar_amd_current_stream());
             if (e) { *status = e; return; }
         }

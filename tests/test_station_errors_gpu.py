@@ -10,8 +10,10 @@ C-ABI:
       * a common gain g and a common apodisation weight a multiply the
         array pattern by g a;
   - oskar_station_analyse sets the apply flags only for non-default values;
-  - a non-zero error scale (the reference's time-variable random errors)
-    makes the beam evaluation return OSKAR_ERR_FUNCTION_NOT_AVAILABLE.
+  - time-variable random errors (gain and phase error scales, the draws of
+    the reference's Philox generator keyed by the station's seed, time
+    index and unique id): the beam equals the oracle's, is the same for the
+    same key and differs for another time index.
 """
 import ctypes as C
 import math
@@ -106,11 +108,13 @@ def test_errors_and_apodisation_match_oracle(hip, prec):
         if prec == "f" else (lambda a: np.asarray(a, np.float64))
     ph = np.radians(ph_deg)
     ap = np.stack([apod.real, apod.imag], 1).ravel()
+    z0 = np.zeros(ne)
+    err = oracle.element_errors(1, 0, 0, f(gain), z0, f(ph), z0)
     want = oracle.station_beam_aa(
         1, *[f(v) for v in xt], *[f(v) for v in xm], np.zeros(ne),
         np.zeros(ne, np.int32), np.array([0], np.int32),
         np.array([0.5 * C0 / FREQ]), azel_dir(az, el), f(x), f(y), f(z),
-        FREQ, gain=f(gain), phase_rad=f(ph), apod=f(ap))
+        FREQ, err=err, apod=f(ap))
     if prec == "d":
         assert max_rel(got, want) < 1e-10
     else:
@@ -169,7 +173,7 @@ def test_common_gain_and_apodisation_scale_the_pattern(hip):
                                atol=1e-12 * np.abs(want).max())
 
 
-def test_analyse_flags_and_random_errors(hip):
+def test_analyse_flags_and_argument_checks(hip):
     rng = np.random.default_rng(44)
     st, _, _ = make("d", 16, rng)
     s = C.c_int(0)
@@ -189,13 +193,63 @@ def test_analyse_flags_and_random_errors(hip):
     s = C.c_int(0)
     lib.oskar_station_set_element_weight(st.ptr, 0, 16, 1.0, 0.0, C.byref(s))
     assert s.value == -5
-    # A random-error scale: the beam is not available.
+
+
+def beam_t(prec, st, x, y, z, time_index):
+    P = OSKAR_DOUBLE if prec == "d" else OSKAR_SINGLE
+    mx, my, mz = (Mem.from_numpy(a, P) for a in (x, y, z))
+    out = Mem(P | OSKAR_COMPLEX | OSKAR_MATRIX, OSKAR_GPU, x.size)
+    work = StationWork(P, OSKAR_GPU)
     s = C.c_int(0)
-    lib.oskar_station_set_element_errors(st.ptr, 0, 5, 1.0, 0.1, 0.0, 0.0,
-                                         C.byref(s))
+    lib.oskar_evaluate_station_beam_aperture_array(
+        st.ptr, work.ptr, x.size, mx.ptr, my.ptr, mz.ptr, int(time_index),
+        0.0, FREQ, out.ptr, C.byref(s))
+    check_status(s, "station beam")
+    sync_check("station beam")
+    return out.numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("prec", ["f", "d"])
+def test_random_element_errors_match_oracle(hip, prec):
+    print("random element errors", prec, flush=True)
+    rng = np.random.default_rng(45)
+    ne = 70                      # odd number of Philox pairs
+    st, xt, xm = make(prec, ne, rng)
+    gain = rng.uniform(0.8, 1.2, ne)
+    gerr = rng.uniform(0.0, 0.1, ne)
+    ph_deg = rng.uniform(-10, 10, ne)
+    perr_deg = rng.uniform(0.0, 5.0, ne)
+    s = C.c_int(0)
+    for i in range(ne):
+        lib.oskar_station_set_element_errors(st.ptr, 0, i, gain[i], gerr[i],
+                                             ph_deg[i], perr_deg[i],
+                                             C.byref(s))
     check_status(s, "set element errors")
+    lib.oskar_station_set_seed_time_variable_errors(st.ptr, 42)
+    counter = C.c_int(7)
+    lib.oskar_station_set_unique_ids(st.ptr, C.byref(counter))
+    assert counter.value == 8 and lib.oskar_station_unique_id(st.ptr) == 7
     analyse(st)
     assert lib.oskar_station_apply_element_errors(st.ptr) == 1
-    lib.oskar_station_set_phase_centre(st.ptr, AZEL, 0.0, math.radians(80))
-    x, y, z = directions(rng, 100)
-    beam("d", st, x, y, z, expect_status=-4)
+    az, el = math.radians(200), math.radians(50)
+    lib.oskar_station_set_phase_centre(st.ptr, AZEL, az, el)
+    x, y, z = directions(rng, 1200)
+    f = (lambda a: np.asarray(a, np.float32).astype(np.float64)) \
+        if prec == "f" else (lambda a: np.asarray(a, np.float64))
+    got3 = beam_t(prec, st, x, y, z, 3)
+    err = oracle.element_errors(42, 3, 7, f(gain), f(gerr),
+                                f(np.radians(ph_deg)),
+                                f(np.radians(perr_deg)))
+    want = oracle.station_beam_aa(
+        1, *[f(v) for v in xt], *[f(v) for v in xm], np.zeros(ne),
+        np.zeros(ne, np.int32), np.array([0], np.int32),
+        np.array([0.5 * C0 / FREQ]), azel_dir(az, el), f(x), f(y), f(z),
+        FREQ, err=err)
+    if prec == "d":
+        assert max_rel(got3, want) < 1e-10
+    else:
+        assert rel_rms(got3, want) < 2e-5
+    # The same key gives the same beam; another time index another one.
+    assert np.array_equal(beam_t(prec, st, x, y, z, 3), got3)
+    got4 = beam_t(prec, st, x, y, z, 4)
+    assert rel_rms(got4, got3) > 1e-3
