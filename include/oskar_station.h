@@ -112,6 +112,27 @@ OSKAR_EXPORT int oskar_station_unique_id(const oskar_Station* station);
 OSKAR_EXPORT void oskar_station_set_unique_ids(oskar_Station* station,
         int* counter);
 
+/* Hierarchical stations, whose elements are child stations:
+ * oskar/telescope/station/src/oskar_station_create_child_stations.c:15-44
+ * (one empty child per element; OSKAR_ERR_MEMORY_ALLOC_FAILURE if children
+ * exist), oskar_station_accessors.c:166-168 and 422-436 (accessors), and
+ * oskar_station_different.c:15-200 (nonzero if two stations differ in
+ * anything but their unique id, children compared recursively).
+ * oskar_station_analyse sets identical_children when every child equals the
+ * first and no station below has random element errors; the aperture-array
+ * beam then evaluates one child and copies it. */
+OSKAR_EXPORT void oskar_station_create_child_stations(oskar_Station* station,
+        int* status);
+OSKAR_EXPORT int oskar_station_has_child(const oskar_Station* station);
+OSKAR_EXPORT oskar_Station* oskar_station_child(oskar_Station* station,
+        int i);
+OSKAR_EXPORT const oskar_Station* oskar_station_child_const(
+        const oskar_Station* station, int i);
+OSKAR_EXPORT int oskar_station_identical_children(
+        const oskar_Station* station);
+OSKAR_EXPORT int oskar_station_different(const oskar_Station* a,
+        const oskar_Station* b, int* status);
+
 /* oskar/telescope/station/src/oskar_station_accessors.c:211-219: set by
  * oskar_station_analyse (any gain != 1 or phase offset != 0 or error scale
  * != 0; any apodisation weight != 1 + 0i). */

@@ -115,6 +115,11 @@ struct oskar_Station
     int num_permitted_beams;          /* private_station.h:91-93 */
     oskar_Mem* permitted_beam_az_rad; /* double, host; NULL until loaded */
     oskar_Mem* permitted_beam_el_rad;
+    /* Child stations, one per element, or NULL (private_station.h child,
+     * identical_children): an element of a hierarchical station is itself
+     * a station whose beam takes the place of an element pattern. */
+    struct oskar_Station** child;
+    int identical_children;
 };
 
 /* oskar/telescope/station/private_station_work.h (subset). */
@@ -131,6 +136,10 @@ struct oskar_StationWork
     oskar_Mem* beam_out;      /* beam of those directions */
     oskar_Mem* horizon_mask;  /* int, one per sky source */
     oskar_Mem* source_indices;/* int, one per sky source + 1 (the total) */
+    /* Child-station beams per recursion depth >= 1 (private_station_work.h
+     * beam[], num_depths); depth 0 is the element-pattern signal above. */
+    oskar_Mem** depth_beam;
+    int num_depths;
 };
 
 /* oskar/sky/private_sky.h: one table of num_columns x capacity values

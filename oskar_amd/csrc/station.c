@@ -156,6 +156,12 @@ void oskar_station_free(oskar_Station* s, int* status)
     oskar_mem_free(s->element_weight, status);
     for (i = 0; i < s->num_element_types; ++i) free(s->element[i]);
     free(s->element);
+    if (s->child)
+    {
+        for (i = 0; i < s->num_elements; ++i)
+            oskar_station_free(s->child[i], status);
+        free(s->child);
+    }
     free(s);
 }
 
@@ -258,7 +264,157 @@ oskar_Station* oskar_station_create_copy(const oskar_Station* src,
                 src->permitted_beam_el_rad, OSKAR_CPU, status);
         s->num_permitted_beams = src->num_permitted_beams;
     }
+    s->identical_children = src->identical_children;
+    if (src->child && !*status)
+    {
+        s->child = (oskar_Station**) calloc((size_t) src->num_elements,
+                sizeof(oskar_Station*));
+        if (!s->child) { *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE; return s; }
+        for (i = 0; i < src->num_elements; ++i)
+            s->child[i] = oskar_station_create_copy(src->child[i], location,
+                    status);
+    }
     return s;
+}
+
+/* ---- Child stations (hierarchical stations) ----------------------------- */
+/* oskar/telescope/station/src/oskar_station_create_child_stations.c:15-44:
+ * one empty child station per element, of the parent's precision and
+ * location; an existing child array is an allocation error. */
+void oskar_station_create_child_stations(oskar_Station* s, int* status)
+{
+    int i = 0;
+    if (*status || !s) return;
+    if (s->child)
+    {
+        *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE;
+        return;
+    }
+    s->child = (oskar_Station**) calloc((size_t) (s->num_elements > 0 ?
+            s->num_elements : 1), sizeof(oskar_Station*));
+    if (!s->child) { *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE; return; }
+    for (i = 0; i < s->num_elements; ++i)
+        s->child[i] = oskar_station_create(s->precision, s->mem_location, 0,
+                status);
+}
+
+/* oskar_station_accessors.c:166-168, 422-436. */
+int oskar_station_has_child(const oskar_Station* s)
+{
+    return s ? (s->child ? 1 : 0) : 0;
+}
+
+oskar_Station* oskar_station_child(oskar_Station* s, int i)
+{
+    return s ? s->child[i] : 0;
+}
+
+const oskar_Station* oskar_station_child_const(const oskar_Station* s, int i)
+{
+    return s ? s->child[i] : 0;
+}
+
+int oskar_station_identical_children(const oskar_Station* s)
+{
+    return s ? s->identical_children : 0;
+}
+
+/* Nonzero if n values of two arrays differ, or their type or length does
+ * (oskar_mem_different for the arrays a station holds; either may be
+ * NULL, which matches only NULL). */
+static int mem_different(const oskar_Mem* a, const oskar_Mem* b, size_t n,
+        int* status)
+{
+    int diff = 0;
+    if (!a || !b) return a != b;
+    if (oskar_mem_type(a) != oskar_mem_type(b)) return 1;
+    if (n == 0)
+    {
+        if (oskar_mem_length(a) != oskar_mem_length(b)) return 1;
+        n = oskar_mem_length(a);
+    }
+    if (oskar_mem_length(a) < n || oskar_mem_length(b) < n) return 1;
+    {
+        const size_t bytes = n * oskar_mem_element_size(oskar_mem_type(a));
+        oskar_Mem* ha = oskar_mem_create_copy(a, OSKAR_CPU, status);
+        oskar_Mem* hb = oskar_mem_create_copy(b, OSKAR_CPU, status);
+        if (!*status)
+            diff = memcmp(oskar_mem_void_const(ha), oskar_mem_void_const(hb),
+                    bytes) != 0;
+        oskar_mem_free(ha, status);
+        oskar_mem_free(hb, status);
+    }
+    return diff;
+}
+
+/* oskar/telescope/station/src/oskar_station_different.c:15-200, over the
+ * fields this library's stations hold: the meta-data, the element arrays
+ * (coordinates, orientations, cable-length errors, gains, phase offsets,
+ * apodisation weights, types), the element models, the permitted beams and,
+ * recursively, the child stations.  The unique ID is not compared. */
+int oskar_station_different(const oskar_Station* a, const oskar_Station* b,
+        int* status)
+{
+    int i = 0, f = 0;
+    size_t n = 0;
+    if (!a || !b) return 0;
+    if (*status) return 1;
+    n = (size_t) a->num_elements;
+    if (a->station_type != b->station_type ||
+            a->normalise_final_beam != b->normalise_final_beam ||
+            a->beam_coord_type != b->beam_coord_type ||
+            a->beam_lon_rad != b->beam_lon_rad ||
+            a->beam_lat_rad != b->beam_lat_rad ||
+            a->identical_children != b->identical_children ||
+            a->num_elements != b->num_elements ||
+            a->num_element_types != b->num_element_types ||
+            a->normalise_array_pattern != b->normalise_array_pattern ||
+            a->normalise_element_pattern != b->normalise_element_pattern ||
+            a->enable_array_pattern != b->enable_array_pattern ||
+            a->common_element_orientation != b->common_element_orientation ||
+            a->common_pol_beams != b->common_pol_beams ||
+            a->array_is_3d != b->array_is_3d ||
+            a->apply_element_errors != b->apply_element_errors ||
+            a->apply_element_weight != b->apply_element_weight ||
+            a->num_permitted_beams != b->num_permitted_beams ||
+            (a->child != 0) != (b->child != 0))
+        return 1;
+    for (i = 0; i < 3; ++i)
+        if (mem_different(a->element_measured_enu_metres[i],
+                b->element_measured_enu_metres[i], n, status) ||
+                mem_different(a->element_true_enu_metres[i],
+                b->element_true_enu_metres[i], n, status))
+            return 1;
+    for (f = 0; f < 2; ++f)
+        for (i = 0; i < 3; ++i)
+            if (mem_different(a->element_euler_cpu[f][i],
+                    b->element_euler_cpu[f][i], n, status))
+                return 1;
+    if (mem_different(a->element_gain, b->element_gain, n, status) ||
+            mem_different(a->element_phase_offset_rad,
+            b->element_phase_offset_rad, n, status) ||
+            mem_different(a->element_weight, b->element_weight, n, status) ||
+            mem_different(a->element_cable_length_error,
+            b->element_cable_length_error, n, status) ||
+            mem_different(a->element_types, b->element_types, n, status) ||
+            mem_different(a->element_types_cpu, b->element_types_cpu, n,
+            status) ||
+            mem_different(a->permitted_beam_az_rad, b->permitted_beam_az_rad,
+            0, status) ||
+            mem_different(a->permitted_beam_el_rad, b->permitted_beam_el_rad,
+            0, status))
+        return 1;
+    for (i = 0; i < a->num_element_types; ++i)
+        if (a->element[i]->element_type != b->element[i]->element_type ||
+                a->element[i]->dipole_length != b->element[i]->dipole_length ||
+                a->element[i]->dipole_length_units !=
+                b->element[i]->dipole_length_units)
+            return 1;
+    if (a->child)
+        for (i = 0; i < a->num_elements; ++i)
+            if (oskar_station_different(a->child[i], b->child[i], status))
+                return 1;
+    return 0;
 }
 
 void oskar_station_resize_element_types(oskar_Station* s,
@@ -403,9 +559,16 @@ int oskar_station_unique_id(const oskar_Station* s)
     return s ? s->unique_id : 0;
 }
 
+/* oskar_station_accessors.c:503-515: depth first, parent before its
+ * children. */
 void oskar_station_set_unique_ids(oskar_Station* s, int* counter)
 {
-    if (s && counter) s->unique_id = (*counter)++;
+    int i = 0;
+    if (!s || !counter) return;
+    s->unique_id = (*counter)++;
+    if (s->child)
+        for (i = 0; i < s->num_elements; ++i)
+            oskar_station_set_unique_ids(s->child[i], counter);
 }
 
 int oskar_station_apply_element_errors(const oskar_Station* s)
@@ -454,7 +617,6 @@ void oskar_station_analyse(oskar_Station* s,
         int* finished_identical_station_check, int* status)
 {
     int f = 0, i = 0, d = 0;
-    (void) finished_identical_station_check;
     if (*status || !s) return;
     s->array_is_3d = 0;
     s->common_element_orientation = 1;
@@ -504,7 +666,13 @@ void oskar_station_analyse(oskar_Station* s,
                 if (oskar_mem_get_element(h, (size_t) i, status) != dflt[d])
                 {
                     s->apply_element_errors = 1;
-                    if (d >= 2) s->element_errors_random = 1;
+                    /* Random errors make every station unique
+                     * (oskar_station_analyse.c:119-125). */
+                    if (d >= 2)
+                    {
+                        s->element_errors_random = 1;
+                        *finished_identical_station_check = 1;
+                    }
                     break;
                 }
             oskar_mem_free(h, status);
@@ -532,6 +700,24 @@ void oskar_station_analyse(oskar_Station* s,
     }
     /* Keep the device copy of the element types in step with the host. */
     oskar_mem_copy(s->element_types, s->element_types_cpu, status);
+
+    /* Children, then whether they are all alike
+     * (oskar_station_analyse.c:193-220). */
+    if (s->child)
+    {
+        for (i = 0; i < s->num_elements; ++i)
+            oskar_station_analyse(s->child[i],
+                    finished_identical_station_check, status);
+        s->identical_children = 0;
+        if (!*finished_identical_station_check)
+        {
+            s->identical_children = 1;
+            for (i = 1; i < s->num_elements; ++i)
+                if (oskar_station_different(s->child[0], s->child[i],
+                        status))
+                    s->identical_children = 0;
+        }
+    }
 }
 
 void oskar_station_set_station_type(oskar_Station* m, int type)
@@ -676,6 +862,12 @@ void oskar_station_work_free(oskar_StationWork* w, int* status)
     oskar_mem_free(w->beam_out, status);
     oskar_mem_free(w->horizon_mask, status);
     oskar_mem_free(w->source_indices, status);
+    {
+        int i = 0;
+        for (i = 0; i < w->num_depths; ++i)
+            oskar_mem_free(w->depth_beam[i], status);
+        free(w->depth_beam);
+    }
     free(w);
 }
 
@@ -1003,7 +1195,7 @@ void oskar_blank_below_horizon(int offset_mask, int num_sources,
 /* oskar_element_evaluate() for dipole and isotropic elements. */
 static void element_evaluate(const oskar_Element* model, int normalise,
         int swap_xy, double orientation_x, double orientation_y,
-        int num_points, const oskar_Mem* x, const oskar_Mem* y,
+        int offset_points, int num_points, const oskar_Mem* x, const oskar_Mem* y,
         const oskar_Mem* z, double frequency_hz, oskar_Mem* theta,
         oskar_Mem* phi_x, oskar_Mem* phi_y, int offset_out, oskar_Mem* output,
         int* status)
@@ -1018,7 +1210,8 @@ static void element_evaluate(const oskar_Element* model, int normalise,
     dipole_length_m = model->dipole_length;
     if (model->dipole_length_units == OSKAR_WAVELENGTHS)
         dipole_length_m *= (C_0 / frequency_hz);
-    oskar_convert_enu_directions_to_theta_phi(0, num_points, x, y, z,
+    oskar_convert_enu_directions_to_theta_phi(offset_points, num_points,
+            x, y, z,
             normalise, (M_PI / 2) - orientation_x, (M_PI / 2) - orientation_y,
             theta, phi_x, phi_y, status);
     if (model->element_type == OSKAR_ELEMENT_TYPE_ISOTROPIC)
@@ -1034,12 +1227,83 @@ static void element_evaluate(const oskar_Element* model, int normalise,
                 (size_t) (offset_out + num_points), status);
 }
 
-void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
-        oskar_StationWork* work, int num_points, const oskar_Mem* x,
+/* The work array for one recursion depth (oskar_station_work_beam,
+ * oskar_station_work.c:256-274): depth 0 holds element patterns, depth d >= 1
+ * the beams of the children of a station at depth d. */
+static oskar_Mem* work_beam(oskar_StationWork* w, const oskar_Mem* beam,
+        size_t length, int depth, int* status)
+{
+    oskar_Mem** m = 0;
+    if (*status) return 0;
+    if (depth == 0) m = &w->signal;
+    else
+    {
+        if (depth >= w->num_depths)
+        {
+            int i = 0;
+            oskar_Mem** b = (oskar_Mem**) realloc(w->depth_beam,
+                    (size_t) (depth + 1) * sizeof(oskar_Mem*));
+            if (!b) { *status = OSKAR_ERR_MEMORY_ALLOC_FAILURE; return 0; }
+            for (i = w->num_depths; i <= depth; ++i) b[i] = 0;
+            w->depth_beam = b;
+            w->num_depths = depth + 1;
+        }
+        m = &w->depth_beam[depth];
+    }
+    if (*m && oskar_mem_type(*m) != oskar_mem_type(beam))
+    {
+        oskar_mem_free(*m, status);
+        *m = 0;
+    }
+    if (!*m) *m = oskar_mem_create(oskar_mem_type(beam), OSKAR_GPU, 0, status);
+    oskar_mem_ensure(*m, length, status);
+    return *m;
+}
+
+/* Element weights of one station for the beam direction, with its element
+ * errors and apodisation (oskar_station_evaluate_element_weights.c:15-66). */
+static void station_weights(const oskar_Station* s, oskar_StationWork* work,
+        double wavenumber, double beam_x, double beam_y, double beam_z,
+        int time_index, int* status)
+{
+    if (*status) return;
+    oskar_mem_ensure(work->weights, (size_t) s->num_elements, status);
+    oskar_evaluate_element_weights_dft(s->num_elements,
+            s->element_measured_enu_metres[0],
+            s->element_measured_enu_metres[1],
+            s->element_measured_enu_metres[2],
+            s->element_cable_length_error, wavenumber, beam_x, beam_y,
+            beam_z, work->weights, status);
+    if ((s->apply_element_errors || s->apply_element_weight) && !*status)
+    {
+        const int err = s->apply_element_errors;
+        const int rnd = err && s->element_errors_random;
+        const int e = oskar_amd_weights_scale_launch(s->precision,
+                s->num_elements, oskar_mem_void(work->weights),
+                err ? oskar_mem_void_const(s->element_gain) : 0,
+                rnd ? oskar_mem_void_const(s->element_gain_error) : 0,
+                err ? oskar_mem_void_const(s->element_phase_offset_rad) : 0,
+                rnd ? oskar_mem_void_const(s->element_phase_error_rad) : 0,
+                s->apply_element_weight ?
+                oskar_mem_void_const(s->element_weight) : 0,
+                s->seed_time_variable_errors, time_index, s->unique_id,
+                oskar_amd_current_stream());
+        if (e) *status = e;
+    }
+}
+
+/* One station of a (possibly hierarchical) aperture array, for points
+ * [offset_points, offset_points + num_points) of x, y, z, into beam from
+ * offset_out (oskar_evaluate_station_beam_aperture_array.c:70-288, the
+ * branches for element types and for child stations). */
+static void aa_beam(oskar_Station* s, oskar_StationWork* work,
+        int offset_points, int num_points, const oskar_Mem* x,
         const oskar_Mem* y, const oskar_Mem* z, int time_index,
-        double gast_rad, double frequency_hz, oskar_Mem* beam, int* status)
+        double gast_rad, double frequency_hz, int depth, int offset_out,
+        oskar_Mem* beam, int* status)
 {
     double beam_x = 0.0, beam_y = 0.0, beam_z = 0.0;
+    oskar_Mem* signal = 0;
     int i = 0;
     if (*status) return;
     if (s->station_type != OSKAR_STATION_TYPE_AA ||
@@ -1050,6 +1314,111 @@ void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
         *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE;
         return;
     }
+    if (s->mem_location != OSKAR_GPU)
+    {
+        *status = OSKAR_ERR_BAD_LOCATION;
+        return;
+    }
+    if (s->precision != oskar_mem_precision(beam))
+    {
+        *status = OSKAR_ERR_TYPE_MISMATCH;
+        return;
+    }
+    const double wavenumber = 2.0 * M_PI * frequency_hz / C_0;
+    oskar_station_beam_horizon_direction(s, gast_rad, &beam_x, &beam_y,
+            &beam_z, status);
+    oskar_mem_ensure(beam, (size_t) (offset_out + num_points), status);
+
+    if (s->child)
+    {
+        /* Each element is a child station: its beam is the element's
+         * signal (:242-286).  Alike children are evaluated once. */
+        if (!s->enable_array_pattern)
+        {
+            *status = OSKAR_ERR_SETTINGS_TELESCOPE;
+            return;
+        }
+        signal = work_beam(work, beam,
+                (size_t) s->num_elements * (size_t) num_points, depth, status);
+        if (s->identical_children && s->num_elements > 0)
+        {
+            aa_beam(s->child[0], work, offset_points, num_points, x, y, z,
+                    time_index, gast_rad, frequency_hz, depth + 1, 0, signal,
+                    status);
+            for (i = 1; i < s->num_elements; ++i)
+                oskar_mem_copy_contents(signal, signal,
+                        (size_t) i * (size_t) num_points, 0,
+                        (size_t) num_points, status);
+        }
+        else
+        {
+            for (i = 0; i < s->num_elements; ++i)
+                aa_beam(s->child[i], work, offset_points, num_points, x, y, z,
+                        time_index, gast_rad, frequency_hz, depth + 1,
+                        i * num_points, signal, status);
+        }
+        station_weights(s, work, wavenumber, beam_x, beam_y, beam_z,
+                time_index, status);
+        oskar_dftw(s->normalise_array_pattern, s->num_elements, wavenumber,
+                work->weights, s->element_true_enu_metres[0],
+                s->element_true_enu_metres[1], s->element_true_enu_metres[2],
+                offset_points, num_points, x, y, s->array_is_3d ? z : 0, 0,
+                signal, 1, 1, offset_out, beam, status);
+        return;
+    }
+
+    /* Element patterns per element type: signal[type][num_points (+1)]. */
+    const int num_types = s->num_element_types;
+    signal = work_beam(work, beam,
+            (size_t) num_types * (size_t) (num_points + 1), 0, status);
+    for (i = 0; i < num_types; ++i)
+    {
+        element_evaluate(s->element[i], s->normalise_element_pattern,
+                s->swap_xy,
+                oskar_station_element_euler_index_rad(s, 0, 0, 0) + M_PI / 2,
+                oskar_station_element_euler_index_rad(s, 1, 0, 0),
+                offset_points, num_points, x, y, z, frequency_hz, work->theta,
+                work->phi_x, work->phi_y, i * num_points, signal, status);
+    }
+    if (s->enable_array_pattern)
+    {
+        station_weights(s, work, wavenumber, beam_x, beam_y, beam_z,
+                time_index, status);
+        oskar_dftw(s->normalise_array_pattern, s->num_elements, wavenumber,
+                work->weights, s->element_true_enu_metres[0],
+                s->element_true_enu_metres[1], s->element_true_enu_metres[2],
+                offset_points, num_points, x, y, s->array_is_3d ? z : 0,
+                s->element_types, signal, 1, 1, offset_out, beam, status);
+    }
+    else
+    {
+        if (num_types == 1)
+            oskar_mem_copy_contents(beam, signal, (size_t) offset_out, 0,
+                    (size_t) num_points, status);
+        else
+        {
+            *status = OSKAR_ERR_SETTINGS_TELESCOPE;
+            return;
+        }
+    }
+    oskar_blank_below_horizon(offset_points, num_points, z, offset_out, beam,
+            status);
+}
+
+/* Points per pass through a hierarchical station: each depth holds one
+ * beam per child for every point of the pass (the reference's
+ * MAX_CHUNK_SIZE, oskar_evaluate_station_beam_aperture_array.c:21, is 49152;
+ * a larger pass suits a GPU with HBM to spare and changes no result, since
+ * every point is evaluated on its own). */
+#define CHILD_PASS_POINTS (1 << 20)
+
+void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
+        oskar_StationWork* work, int num_points, const oskar_Mem* x,
+        const oskar_Mem* y, const oskar_Mem* z, int time_index,
+        double gast_rad, double frequency_hz, oskar_Mem* beam, int* status)
+{
+    int start = 0;
+    if (*status) return;
     if (oskar_mem_location(beam) != OSKAR_GPU ||
             s->mem_location != OSKAR_GPU || work->mem_location != OSKAR_GPU)
     {
@@ -1062,72 +1431,18 @@ void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
         *status = OSKAR_ERR_TYPE_MISMATCH;
         return;
     }
-    const double wavenumber = 2.0 * M_PI * frequency_hz / C_0;
-    const int num_types = s->num_element_types;
-
-    oskar_station_beam_horizon_direction(s, gast_rad, &beam_x, &beam_y,
-            &beam_z, status);
-
-    /* Element patterns per element type: signal[type][num_points (+1)]. */
-    if (!work->signal || oskar_mem_type(work->signal) != oskar_mem_type(beam))
+    if (!s->child)
     {
-        oskar_mem_free(work->signal, status);
-        work->signal = oskar_mem_create(oskar_mem_type(beam), OSKAR_GPU, 0,
-                status);
+        aa_beam(s, work, 0, num_points, x, y, z, time_index, gast_rad,
+                frequency_hz, 0, 0, beam, status);
+        return;
     }
-    oskar_mem_ensure(work->signal,
-            (size_t) num_types * (size_t) (num_points + 1), status);
-    for (i = 0; i < num_types; ++i)
+    /* Depth 0 is the element level, so the recursion starts at 1 (:52-66). */
+    for (start = 0; start < num_points; start += CHILD_PASS_POINTS)
     {
-        element_evaluate(s->element[i], s->normalise_element_pattern,
-                s->swap_xy,
-                oskar_station_element_euler_index_rad(s, 0, 0, 0) + M_PI / 2,
-                oskar_station_element_euler_index_rad(s, 1, 0, 0),
-                num_points, x, y, z, frequency_hz, work->theta, work->phi_x,
-                work->phi_y, i * num_points, work->signal, status);
+        const int n = (num_points - start < CHILD_PASS_POINTS) ?
+                num_points - start : CHILD_PASS_POINTS;
+        aa_beam(s, work, start, n, x, y, z, time_index, gast_rad,
+                frequency_hz, 1, start, beam, status);
     }
-    oskar_mem_ensure(beam, (size_t) num_points, status);
-    if (s->enable_array_pattern)
-    {
-        oskar_mem_ensure(work->weights, (size_t) s->num_elements, status);
-        oskar_evaluate_element_weights_dft(s->num_elements,
-                s->element_measured_enu_metres[0],
-                s->element_measured_enu_metres[1],
-                s->element_measured_enu_metres[2],
-                s->element_cable_length_error, wavenumber, beam_x, beam_y,
-                beam_z, work->weights, status);
-        if ((s->apply_element_errors || s->apply_element_weight) && !*status)
-        {
-            const int err = s->apply_element_errors;
-            const int rnd = err && s->element_errors_random;
-            const int e = oskar_amd_weights_scale_launch(s->precision,
-                    s->num_elements, oskar_mem_void(work->weights),
-                    err ? oskar_mem_void_const(s->element_gain) : 0,
-                    rnd ? oskar_mem_void_const(s->element_gain_error) : 0,
-                    err ? oskar_mem_void_const(s->element_phase_offset_rad) : 0,
-                    rnd ? oskar_mem_void_const(s->element_phase_error_rad) : 0,
-                    s->apply_element_weight ?
-                    oskar_mem_void_const(s->element_weight) : 0,
-                    s->seed_time_variable_errors, time_index, s->unique_id,
-                    oskar_amd_current_stream());
-            if (e) { *status = e; return; }
-        }
-        oskar_dftw(s->normalise_array_pattern, s->num_elements, wavenumber,
-                work->weights, s->element_true_enu_metres[0],
-                s->element_true_enu_metres[1], s->element_true_enu_metres[2],
-                0, num_points, x, y, s->array_is_3d ? z : 0, s->element_types,
-                work->signal, 1, 1, 0, beam, status);
-    }
-    else
-    {
-        if (num_types == 1)
-            oskar_mem_copy_contents(beam, work->signal, 0, 0,
-                    (size_t) num_points, status);
-        else
-        {
-            *status = OSKAR_ERR_SETTINGS_TELESCOPE;
-            return;
-        }
-    }
-    oskar_blank_below_horizon(0, num_points, z, 0, beam, status);
 }

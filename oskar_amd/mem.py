@@ -226,6 +226,18 @@ class Station:
             pass
 
 
+class StationView(Station):
+    """Non-owning handle to a station held by another (a child station):
+    freed with its parent."""
+
+    def __init__(self, ptr, precision):
+        self.ptr = ptr
+        self.precision = precision
+
+    def __del__(self):
+        pass
+
+
 class StationWork:
     def __init__(self, precision, location):
         st = C.c_int(0)
