@@ -1306,11 +1306,10 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
     oskar_Mem* signal = 0;
     int i = 0;
     if (*status) return;
-    if (s->station_type != OSKAR_STATION_TYPE_AA ||
-            !s->common_element_orientation || !s->common_pol_beams)
+    if (s->station_type != OSKAR_STATION_TYPE_AA || !s->common_pol_beams)
     {
-        /* Per-element orientations, separate polarisation beams and other
-         * station kinds are not on this library's path. */
+        /* Separate polarisation beams (feed-specific element data) and
+         * other station kinds are not on this library's path. */
         *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE;
         return;
     }
@@ -1367,18 +1366,51 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
         return;
     }
 
-    /* Element patterns per element type: signal[type][num_points (+1)]. */
     const int num_types = s->num_element_types;
-    signal = work_beam(work, beam,
-            (size_t) num_types * (size_t) (num_points + 1), 0, status);
-    for (i = 0; i < num_types; ++i)
+    const oskar_Mem* data_idx = 0;
+    if (s->common_element_orientation)
     {
-        element_evaluate(s->element[i], s->normalise_element_pattern,
-                s->swap_xy,
-                oskar_station_element_euler_index_rad(s, 0, 0, 0) + M_PI / 2,
-                oskar_station_element_euler_index_rad(s, 1, 0, 0),
-                offset_points, num_points, x, y, z, frequency_hz, work->theta,
-                work->phi_x, work->phi_y, i * num_points, signal, status);
+        /* Element patterns per element type: signal[type][num_points (+1)],
+         * the DFT picking each element's row by its type (:157-177). */
+        data_idx = s->element_types;
+        signal = work_beam(work, beam,
+                (size_t) num_types * (size_t) (num_points + 1), 0, status);
+        for (i = 0; i < num_types; ++i)
+        {
+            element_evaluate(s->element[i], s->normalise_element_pattern,
+                    s->swap_xy,
+                    oskar_station_element_euler_index_rad(s, 0, 0, 0) +
+                    M_PI / 2, oskar_station_element_euler_index_rad(s, 1, 0, 0),
+                    offset_points, num_points, x, y, z, frequency_hz,
+                    work->theta, work->phi_x, work->phi_y, i * num_points,
+                    signal, status);
+        }
+    }
+    else
+    {
+        /* Elements with their own orientations: one pattern per element,
+         * of its type, turned by its own Euler angles; row j of the signal
+         * is element j's (:178-201). */
+        const int* type = (const int*) oskar_mem_void_const(
+                s->element_types_cpu);
+        signal = work_beam(work, beam,
+                (size_t) s->num_elements * (size_t) (num_points + 1), 0,
+                status);
+        for (i = 0; i < s->num_elements && !*status; ++i)
+        {
+            if (type[i] < 0 || type[i] >= num_types)
+            {
+                *status = OSKAR_ERR_OUT_OF_RANGE;
+                break;
+            }
+            element_evaluate(s->element[type[i]], s->normalise_element_pattern,
+                    s->swap_xy,
+                    oskar_station_element_euler_index_rad(s, 0, 0, i) +
+                    M_PI / 2, oskar_station_element_euler_index_rad(s, 1, 0, i),
+                    offset_points, num_points, x, y, z, frequency_hz,
+                    work->theta, work->phi_x, work->phi_y, i * num_points,
+                    signal, status);
+        }
     }
     if (s->enable_array_pattern)
     {
@@ -1388,10 +1420,12 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
                 work->weights, s->element_true_enu_metres[0],
                 s->element_true_enu_metres[1], s->element_true_enu_metres[2],
                 offset_points, num_points, x, y, s->array_is_3d ? z : 0,
-                s->element_types, signal, 1, 1, offset_out, beam, status);
+                data_idx, signal, 1, 1, offset_out, beam, status);
     }
     else
     {
+        /* Without the array pattern the beam is the first row's pattern
+         * (:224-229). */
         if (num_types == 1)
             oskar_mem_copy_contents(beam, signal, (size_t) offset_out, 0,
                     (size_t) num_points, status);
