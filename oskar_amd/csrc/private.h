@@ -120,6 +120,17 @@ struct oskar_Station
      * a station whose beam takes the place of an element pattern. */
     struct oskar_Station** child;
     int identical_children;
+    /* Feed-1 (Y) element data (private_station.h arrays [1]), NULL until set
+     * for feed 1: while all are NULL the two feeds share feed 0's data and
+     * the station has common polarisation beams. */
+    oskar_Mem* y_true_enu_metres[3];
+    oskar_Mem* y_measured_enu_metres[3];
+    oskar_Mem* y_cable_length_error;
+    oskar_Mem* y_gain;
+    oskar_Mem* y_gain_error;
+    oskar_Mem* y_phase_offset_rad;
+    oskar_Mem* y_phase_error_rad;
+    oskar_Mem* y_weight;
 };
 
 /* oskar/telescope/station/private_station_work.h (subset). */

@@ -76,6 +76,46 @@ int oskar_element_dipole_length_units(const oskar_Element* d)
 { return d->dipole_length_units; }
 
 /* ---- Station ------------------------------------------------------------ */
+/* Per-feed element arrays: slot(s, feed, f) is the field's array for that
+ * feed; feed_mem falls back to feed 0 where feed 1 has none
+ * (oskar_station_accessors.c:259-275). */
+enum { F_TRUE = 0, F_MEAS = 3, F_CABLE = 6, F_GAIN, F_GAIN_ERR, F_PHASE,
+    F_PHASE_ERR, F_WEIGHT, F_COUNT };
+
+static oskar_Mem** slot(oskar_Station* s, int feed, int f)
+{
+    if (f < F_MEAS) return feed ? &s->y_true_enu_metres[f] :
+            &s->element_true_enu_metres[f];
+    if (f < F_CABLE) return feed ? &s->y_measured_enu_metres[f - F_MEAS] :
+            &s->element_measured_enu_metres[f - F_MEAS];
+    switch (f)
+    {
+    case F_CABLE: return feed ? &s->y_cable_length_error :
+            &s->element_cable_length_error;
+    case F_GAIN: return feed ? &s->y_gain : &s->element_gain;
+    case F_GAIN_ERR: return feed ? &s->y_gain_error : &s->element_gain_error;
+    case F_PHASE: return feed ? &s->y_phase_offset_rad :
+            &s->element_phase_offset_rad;
+    case F_PHASE_ERR: return feed ? &s->y_phase_error_rad :
+            &s->element_phase_error_rad;
+    default: return feed ? &s->y_weight : &s->element_weight;
+    }
+}
+
+static oskar_Mem* feed_mem(const oskar_Station* s, int feed, int f)
+{
+    oskar_Mem* m = *slot((oskar_Station*) s, feed ? 1 : 0, f);
+    return m ? m : *slot((oskar_Station*) s, 0, f);
+}
+
+static int has_feed1_data(const oskar_Station* s)
+{
+    int f = 0;
+    for (f = 0; f < F_COUNT; ++f)
+        if (*slot((oskar_Station*) s, 1, f)) return 1;
+    return 0;
+}
+
 oskar_Station* oskar_station_create(int type, int location, int num_elements,
         int* status)
 {
@@ -154,6 +194,7 @@ void oskar_station_free(oskar_Station* s, int* status)
     oskar_mem_free(s->element_phase_offset_rad, status);
     oskar_mem_free(s->element_phase_error_rad, status);
     oskar_mem_free(s->element_weight, status);
+    for (f = 0; f < F_COUNT; ++f) oskar_mem_free(*slot(s, 1, f), status);
     for (i = 0; i < s->num_element_types; ++i) free(s->element[i]);
     free(s->element);
     if (s->child)
@@ -194,6 +235,8 @@ void oskar_station_resize(oskar_Station* s, int num_elements, int* status)
             resize_zero(s->element_euler_cpu[f][i], num_elements, status);
     resize_zero(s->element_types, num_elements, status);
     resize_zero(s->element_types_cpu, num_elements, status);
+    for (f = 0; f < F_COUNT; ++f)
+        if (*slot(s, 1, f)) resize_zero(*slot(s, 1, f), num_elements, status);
     s->num_elements = num_elements;
 }
 
@@ -264,6 +307,10 @@ oskar_Station* oskar_station_create_copy(const oskar_Station* src,
                 src->permitted_beam_el_rad, OSKAR_CPU, status);
         s->num_permitted_beams = src->num_permitted_beams;
     }
+    for (f = 0; f < F_COUNT && !*status; ++f)
+        if (*slot((oskar_Station*) src, 1, f))
+            *slot(s, 1, f) = oskar_mem_create_copy(
+                    *slot((oskar_Station*) src, 1, f), location, status);
     s->identical_children = src->identical_children;
     if (src->child && !*status)
     {
@@ -404,6 +451,10 @@ int oskar_station_different(const oskar_Station* a, const oskar_Station* b,
             mem_different(a->permitted_beam_el_rad, b->permitted_beam_el_rad,
             0, status))
         return 1;
+    for (f = 0; f < F_COUNT; ++f)
+        if (mem_different(*slot((oskar_Station*) a, 1, f),
+                *slot((oskar_Station*) b, 1, f), n, status))
+            return 1;
     for (i = 0; i < a->num_element_types; ++i)
         if (a->element[i]->element_type != b->element[i]->element_type ||
                 a->element[i]->dipole_length != b->element[i]->dipole_length ||
@@ -447,40 +498,6 @@ static void set_real(oskar_Mem* m, int index, double v, int* status)
     oskar_mem_set_element_real(m, (size_t) index, v, status);
 }
 
-void oskar_station_set_element_coords(oskar_Station* s, int feed, int index,
-        const double measured_enu[3], const double true_enu[3], int* status)
-{
-    int i = 0;
-    if (*status || !s) return;
-    if (index < 0 || index >= s->num_elements || feed != 0)
-    {
-        /* Only one feed of coordinates (shared by both polarisations). */
-        *status = (feed != 0) ? OSKAR_ERR_FUNCTION_NOT_AVAILABLE :
-                OSKAR_ERR_OUT_OF_RANGE;
-        return;
-    }
-    for (i = 0; i < 3; ++i)
-    {
-        set_real(s->element_measured_enu_metres[i], index, measured_enu[i],
-                status);
-        set_real(s->element_true_enu_metres[i], index, true_enu[i], status);
-    }
-    if (measured_enu[2] != 0.0 || true_enu[2] != 0.0) s->array_is_3d = 1;
-}
-
-void oskar_station_set_element_cable_length_error(oskar_Station* s, int feed,
-        int index, const double error_metres, int* status)
-{
-    if (*status || !s) return;
-    if (feed != 0) { *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE; return; }
-    if (index < 0 || index >= s->num_elements)
-    {
-        *status = OSKAR_ERR_OUT_OF_RANGE;
-        return;
-    }
-    set_real(s->element_cable_length_error, index, error_metres, status);
-}
-
 /* A per-element array of the station, created on first use with every
  * element at the default value (real part; imaginary parts zero). */
 static oskar_Mem* element_array(oskar_Station* s, oskar_Mem** m, int type,
@@ -496,39 +513,75 @@ static oskar_Mem* element_array(oskar_Station* s, oskar_Mem** m, int type,
     return *m;
 }
 
+/* The array of field f for one feed, created on first use.  Feed 1's
+ * starts as a copy of feed 0's (or at the default where feed 0 has none),
+ * so elements not set for feed 1 keep feed 0's values; the reference
+ * leaves them uninitialised (oskar_station_set_element_coords.c:41-54). */
+static oskar_Mem* field_array(oskar_Station* s, int feed, int f, int type,
+        double value, int* status)
+{
+    oskar_Mem** m = slot(s, feed, f);
+    if (feed && !*m && !*status && *slot(s, 0, f))
+        *m = oskar_mem_create_copy(*slot(s, 0, f), s->mem_location, status);
+    return element_array(s, m, type, value, status);
+}
+
+static int bad_feed_or_index(const oskar_Station* s, int feed, int index,
+        int* status)
+{
+    if (*status || !s) return 1;
+    if (feed < 0 || feed > 1 || index < 0 || index >= s->num_elements)
+    {
+        *status = OSKAR_ERR_OUT_OF_RANGE;
+        return 1;
+    }
+    return 0;
+}
+
+void oskar_station_set_element_coords(oskar_Station* s, int feed, int index,
+        const double measured_enu[3], const double true_enu[3], int* status)
+{
+    int i = 0;
+    if (bad_feed_or_index(s, feed, index, status)) return;
+    for (i = 0; i < 3; ++i)
+    {
+        set_real(field_array(s, feed, F_MEAS + i, s->precision, 0.0, status),
+                index, measured_enu[i], status);
+        set_real(field_array(s, feed, F_TRUE + i, s->precision, 0.0, status),
+                index, true_enu[i], status);
+    }
+    if (measured_enu[2] != 0.0 || true_enu[2] != 0.0) s->array_is_3d = 1;
+}
+
+void oskar_station_set_element_cable_length_error(oskar_Station* s, int feed,
+        int index, const double error_metres, int* status)
+{
+    if (bad_feed_or_index(s, feed, index, status)) return;
+    set_real(field_array(s, feed, F_CABLE, s->precision, 0.0, status), index,
+            error_metres, status);
+}
+
 void oskar_station_set_element_errors(oskar_Station* s, int feed, int index,
         double gain, double gain_error, double phase_offset_deg,
         double phase_error_deg, int* status)
 {
-    if (*status || !s) return;
-    if (feed != 0) { *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE; return; }
-    if (index < 0 || index >= s->num_elements)
-    {
-        *status = OSKAR_ERR_OUT_OF_RANGE;
-        return;
-    }
+    if (bad_feed_or_index(s, feed, index, status)) return;
     const int t = s->precision;
-    set_real(element_array(s, &s->element_gain, t, 1.0, status), index, gain,
+    set_real(field_array(s, feed, F_GAIN, t, 1.0, status), index, gain,
             status);
-    set_real(element_array(s, &s->element_gain_error, t, 0.0, status), index,
+    set_real(field_array(s, feed, F_GAIN_ERR, t, 0.0, status), index,
             gain_error, status);
-    set_real(element_array(s, &s->element_phase_offset_rad, t, 0.0, status),
-            index, phase_offset_deg * M_PI / 180.0, status);
-    set_real(element_array(s, &s->element_phase_error_rad, t, 0.0, status),
-            index, phase_error_deg * M_PI / 180.0, status);
+    set_real(field_array(s, feed, F_PHASE, t, 0.0, status), index,
+            phase_offset_deg * M_PI / 180.0, status);
+    set_real(field_array(s, feed, F_PHASE_ERR, t, 0.0, status), index,
+            phase_error_deg * M_PI / 180.0, status);
 }
 
 void oskar_station_set_element_weight(oskar_Station* s, int feed, int index,
         double re, double im, int* status)
 {
-    if (*status || !s) return;
-    if (feed != 0) { *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE; return; }
-    if (index < 0 || index >= s->num_elements)
-    {
-        *status = OSKAR_ERR_OUT_OF_RANGE;
-        return;
-    }
-    oskar_Mem* w = element_array(s, &s->element_weight,
+    if (bad_feed_or_index(s, feed, index, status)) return;
+    oskar_Mem* w = field_array(s, feed, F_WEIGHT,
             s->precision | OSKAR_COMPLEX, 1.0, status);
     if (*status) return;
     {
@@ -634,9 +687,20 @@ void oskar_station_analyse(oskar_Station* s,
                 break;
             }
     }
+    /* Feed-1 data of any kind give separate polarisation beams, and both
+     * feeds are then analysed (oskar_station_analyse.c:62-79). */
+    if (has_feed1_data(s)) s->common_pol_beams = 0;
+    const int num_feeds = s->common_pol_beams ? 1 : 2;
+    /* Element errors and apodisation (oskar_station_analyse.c:80-133):
+     * errors apply if any gain != 1 or phase offset != 0, or any error
+     * scale != 0; apodisation if any weight != 1 + 0i. */
+    s->apply_element_errors = 0;
+    s->element_errors_random = 0;
+    s->apply_element_weight = 0;
+    for (f = 0; f < num_feeds && !*status; ++f)
     {
-        const oskar_Mem* z[2] = { s->element_true_enu_metres[2],
-                s->element_measured_enu_metres[2] };
+        const oskar_Mem* z[2] = { feed_mem(s, f, F_TRUE + 2),
+                feed_mem(s, f, F_MEAS + 2) };
         for (d = 0; d < 2 && !*status; ++d)
         {
             oskar_Mem* h = oskar_mem_create_copy(z[d], OSKAR_CPU, status);
@@ -647,16 +711,9 @@ void oskar_station_analyse(oskar_Station* s,
             }
             oskar_mem_free(h, status);
         }
-    }
-    /* Element errors and apodisation (oskar_station_analyse.c:110-133):
-     * errors apply if any gain != 1 or phase offset != 0, or any error
-     * scale != 0; apodisation if any weight != 1 + 0i. */
-    s->apply_element_errors = 0;
-    s->element_errors_random = 0;
-    s->apply_element_weight = 0;
-    {
-        const oskar_Mem* m[4] = {s->element_gain, s->element_phase_offset_rad,
-                s->element_gain_error, s->element_phase_error_rad};
+        const oskar_Mem* m[4] = {feed_mem(s, f, F_GAIN),
+                feed_mem(s, f, F_PHASE), feed_mem(s, f, F_GAIN_ERR),
+                feed_mem(s, f, F_PHASE_ERR)};
         const double dflt[4] = {1.0, 0.0, 0.0, 0.0};
         for (d = 0; d < 4 && !*status; ++d)
         {
@@ -677,10 +734,10 @@ void oskar_station_analyse(oskar_Station* s,
                 }
             oskar_mem_free(h, status);
         }
-        if (s->element_weight && !*status)
+        const oskar_Mem* wm = feed_mem(s, f, F_WEIGHT);
+        if (wm && !*status)
         {
-            oskar_Mem* h = oskar_mem_create_copy(s->element_weight, OSKAR_CPU,
-                    status);
+            oskar_Mem* h = oskar_mem_create_copy(wm, OSKAR_CPU, status);
             const int dbl = s->precision == OSKAR_DOUBLE;
             const void* p = oskar_mem_void_const(h);
             for (i = 0; i < s->num_elements && !*status; ++i)
@@ -778,26 +835,26 @@ int oskar_station_beam_coord_type(const oskar_Station* m)
 { return m->beam_coord_type; }
 oskar_Mem* oskar_station_element_true_enu_metres(oskar_Station* m, int feed,
         int dim)
-{ return (feed == 0 && dim >= 0 && dim < 3) ?
-        m->element_true_enu_metres[dim] : 0; }
+{ return (feed >= 0 && feed <= 1 && dim >= 0 && dim < 3) ?
+        feed_mem(m, feed, F_TRUE + dim) : 0; }
 const oskar_Mem* oskar_station_element_true_enu_metres_const(
         const oskar_Station* m, int feed, int dim)
-{ return (feed == 0 && dim >= 0 && dim < 3) ?
-        m->element_true_enu_metres[dim] : 0; }
+{ return (feed >= 0 && feed <= 1 && dim >= 0 && dim < 3) ?
+        feed_mem(m, feed, F_TRUE + dim) : 0; }
 oskar_Mem* oskar_station_element_measured_enu_metres(oskar_Station* m,
         int feed, int dim)
-{ return (feed == 0 && dim >= 0 && dim < 3) ?
-        m->element_measured_enu_metres[dim] : 0; }
+{ return (feed >= 0 && feed <= 1 && dim >= 0 && dim < 3) ?
+        feed_mem(m, feed, F_MEAS + dim) : 0; }
 const oskar_Mem* oskar_station_element_measured_enu_metres_const(
         const oskar_Station* m, int feed, int dim)
-{ return (feed == 0 && dim >= 0 && dim < 3) ?
-        m->element_measured_enu_metres[dim] : 0; }
+{ return (feed >= 0 && feed <= 1 && dim >= 0 && dim < 3) ?
+        feed_mem(m, feed, F_MEAS + dim) : 0; }
 oskar_Mem* oskar_station_element_cable_length_error_metres(oskar_Station* m,
         int feed)
-{ return feed == 0 ? m->element_cable_length_error : 0; }
+{ return (feed >= 0 && feed <= 1) ? feed_mem(m, feed, F_CABLE) : 0; }
 const oskar_Mem* oskar_station_element_cable_length_error_metres_const(
         const oskar_Station* m, int feed)
-{ return feed == 0 ? m->element_cable_length_error : 0; }
+{ return (feed >= 0 && feed <= 1) ? feed_mem(m, feed, F_CABLE) : 0; }
 oskar_Mem* oskar_station_element_types(oskar_Station* m)
 { return m->element_types; }
 const oskar_Mem* oskar_station_element_types_const(const oskar_Station* m)
@@ -1262,30 +1319,28 @@ static oskar_Mem* work_beam(oskar_StationWork* w, const oskar_Mem* beam,
 
 /* Element weights of one station for the beam direction, with its element
  * errors and apodisation (oskar_station_evaluate_element_weights.c:15-66). */
-static void station_weights(const oskar_Station* s, oskar_StationWork* work,
-        double wavenumber, double beam_x, double beam_y, double beam_z,
-        int time_index, int* status)
+static void station_weights(const oskar_Station* s, int feed,
+        oskar_StationWork* work, double wavenumber, double beam_x,
+        double beam_y, double beam_z, int time_index, int* status)
 {
     if (*status) return;
     oskar_mem_ensure(work->weights, (size_t) s->num_elements, status);
     oskar_evaluate_element_weights_dft(s->num_elements,
-            s->element_measured_enu_metres[0],
-            s->element_measured_enu_metres[1],
-            s->element_measured_enu_metres[2],
-            s->element_cable_length_error, wavenumber, beam_x, beam_y,
-            beam_z, work->weights, status);
+            feed_mem(s, feed, F_MEAS), feed_mem(s, feed, F_MEAS + 1),
+            feed_mem(s, feed, F_MEAS + 2), feed_mem(s, feed, F_CABLE),
+            wavenumber, beam_x, beam_y, beam_z, work->weights, status);
     if ((s->apply_element_errors || s->apply_element_weight) && !*status)
     {
         const int err = s->apply_element_errors;
         const int rnd = err && s->element_errors_random;
         const int e = oskar_amd_weights_scale_launch(s->precision,
                 s->num_elements, oskar_mem_void(work->weights),
-                err ? oskar_mem_void_const(s->element_gain) : 0,
-                rnd ? oskar_mem_void_const(s->element_gain_error) : 0,
-                err ? oskar_mem_void_const(s->element_phase_offset_rad) : 0,
-                rnd ? oskar_mem_void_const(s->element_phase_error_rad) : 0,
+                err ? oskar_mem_void_const(feed_mem(s, feed, F_GAIN)) : 0,
+                rnd ? oskar_mem_void_const(feed_mem(s, feed, F_GAIN_ERR)) : 0,
+                err ? oskar_mem_void_const(feed_mem(s, feed, F_PHASE)) : 0,
+                rnd ? oskar_mem_void_const(feed_mem(s, feed, F_PHASE_ERR)) : 0,
                 s->apply_element_weight ?
-                oskar_mem_void_const(s->element_weight) : 0,
+                oskar_mem_void_const(feed_mem(s, feed, F_WEIGHT)) : 0,
                 s->seed_time_variable_errors, time_index, s->unique_id,
                 oskar_amd_current_stream());
         if (e) *status = e;
@@ -1306,10 +1361,9 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
     oskar_Mem* signal = 0;
     int i = 0;
     if (*status) return;
-    if (s->station_type != OSKAR_STATION_TYPE_AA || !s->common_pol_beams)
+    if (s->station_type != OSKAR_STATION_TYPE_AA)
     {
-        /* Separate polarisation beams (feed-specific element data) and
-         * other station kinds are not on this library's path. */
+        /* Other station kinds are not on this library's path. */
         *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE;
         return;
     }
@@ -1324,6 +1378,12 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
         return;
     }
     const double wavenumber = 2.0 * M_PI * frequency_hz / C_0;
+    /* Separate beams for the X and Y feeds when the station holds feed-1
+     * data and the beam is polarised (:89-90): each feed's DFT fills its
+     * own row of the Jones matrix. */
+    const int num_feeds = (s->common_pol_beams || !oskar_mem_is_matrix(beam)) ?
+            1 : 2;
+    int feed = 0;
     oskar_station_beam_horizon_direction(s, gast_rad, &beam_x, &beam_y,
             &beam_z, status);
     oskar_mem_ensure(beam, (size_t) (offset_out + num_points), status);
@@ -1356,13 +1416,18 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
                         time_index, gast_rad, frequency_hz, depth + 1,
                         i * num_points, signal, status);
         }
-        station_weights(s, work, wavenumber, beam_x, beam_y, beam_z,
-                time_index, status);
-        oskar_dftw(s->normalise_array_pattern, s->num_elements, wavenumber,
-                work->weights, s->element_true_enu_metres[0],
-                s->element_true_enu_metres[1], s->element_true_enu_metres[2],
-                offset_points, num_points, x, y, s->array_is_3d ? z : 0, 0,
-                signal, 1, 1, offset_out, beam, status);
+        for (feed = 0; feed < num_feeds; ++feed)
+        {
+            station_weights(s, feed, work, wavenumber, beam_x, beam_y,
+                    beam_z, time_index, status);
+            oskar_dftw(s->normalise_array_pattern, s->num_elements,
+                    wavenumber, work->weights, feed_mem(s, feed, F_TRUE),
+                    feed_mem(s, feed, F_TRUE + 1),
+                    feed_mem(s, feed, F_TRUE + 2), offset_points, num_points,
+                    x, y, s->array_is_3d ? z : 0, 0, signal,
+                    feed == 0 || num_feeds == 1, feed == 1 || num_feeds == 1,
+                    offset_out, beam, status);
+        }
         return;
     }
 
@@ -1414,13 +1479,18 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
     }
     if (s->enable_array_pattern)
     {
-        station_weights(s, work, wavenumber, beam_x, beam_y, beam_z,
-                time_index, status);
-        oskar_dftw(s->normalise_array_pattern, s->num_elements, wavenumber,
-                work->weights, s->element_true_enu_metres[0],
-                s->element_true_enu_metres[1], s->element_true_enu_metres[2],
-                offset_points, num_points, x, y, s->array_is_3d ? z : 0,
-                data_idx, signal, 1, 1, offset_out, beam, status);
+        for (feed = 0; feed < num_feeds; ++feed)
+        {
+            station_weights(s, feed, work, wavenumber, beam_x, beam_y,
+                    beam_z, time_index, status);
+            oskar_dftw(s->normalise_array_pattern, s->num_elements,
+                    wavenumber, work->weights, feed_mem(s, feed, F_TRUE),
+                    feed_mem(s, feed, F_TRUE + 1),
+                    feed_mem(s, feed, F_TRUE + 2), offset_points, num_points,
+                    x, y, s->array_is_3d ? z : 0, data_idx, signal,
+                    feed == 0 || num_feeds == 1, feed == 1 || num_feeds == 1,
+                    offset_out, beam, status);
+        }
     }
     else
     {

@@ -10,6 +10,7 @@ C-ABI:
       * a common gain g and a common apodisation weight a multiply the
         array pattern by g a;
   - oskar_station_analyse sets the apply flags only for non-default values;
+    a feed other than 0 or 1, or an element index out of range, is refused;
   - time-variable random errors (gain and phase error scales, the draws of
     the reference's Philox generator keyed by the station's seed, time
     index and unique id): the beam equals the oracle's, is the same for the
@@ -187,9 +188,9 @@ def test_analyse_flags_and_argument_checks(hip):
     analyse(st)
     assert lib.oskar_station_apply_element_errors(st.ptr) == 0
     assert lib.oskar_station_apply_element_weight(st.ptr) == 0
-    # Feed 1 and an index out of range are rejected.
-    lib.oskar_station_set_element_weight(st.ptr, 1, 0, 1.0, 0.0, C.byref(s))
-    assert s.value == -4
+    # Feed 2 and an index out of range are rejected (OSKAR_ERR_OUT_OF_RANGE).
+    lib.oskar_station_set_element_weight(st.ptr, 2, 0, 1.0, 0.0, C.byref(s))
+    assert s.value == -5
     s = C.c_int(0)
     lib.oskar_station_set_element_weight(st.ptr, 0, 16, 1.0, 0.0, C.byref(s))
     assert s.value == -5
