@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--dftw-steps", type=int, default=5,
                     help="oskar_dftw calls at 4096 elements x 1M directions "
                          "(0 = skip)")
+    ap.add_argument("--tile-steps", type=int, default=5,
+                    help="beams of a 16-tile x 16-dipole hierarchical "
+                         "station at 1M directions (0 = skip)")
     ap.add_argument("--direct-channels", type=int, default=16,
                     help="channels of the 512 x 100k Gaussian + smearing "
                          "direct-correlator leg (0 = skip)")
@@ -432,6 +435,9 @@ def main():
     dftw = None
     if args.dftw_steps > 0:
         dftw = bench_dftw(args, rng)
+    tiles = None
+    if args.tile_steps > 0:
+        tiles = bench_tiled_station(args, rng)
 
     # --- batched (channel, time) units in one persistent launch ---------
     batch = None
@@ -528,6 +534,7 @@ def main():
         "sky_parameters": sky_params,
         "direct_gaussian_smearing": direct,
         "dftw": dftw,
+        "tiled_station": tiles,
         "vis_block": vis,
     }
     print(json.dumps(line), flush=True)
@@ -604,6 +611,93 @@ def bench_clip(args, rng):
             "achieved_GBps": round(gbps, 1),
             "hbm_peak_GBps": PEAK_HBM_GBPS,
             "frac": round(gbps / PEAK_HBM_GBPS, 4)}
+
+
+def bench_tiled_station(args, rng):
+    """oskar_evaluate_station_beam_aperture_array on a hierarchical station:
+    16 alike tiles (child stations) of 16 half-wave dipoles on a 1.25 m grid,
+    tile centres within 18 m, FP32 2x2 beam at 1M directions over the upper
+    hemisphere, 150 MHz, pointed at az 30 deg, el 60 deg.  The alike tiles'
+    beam is evaluated once and copied, then the station's DFT runs over the
+    tiles.  Unit: one direction of the station beam.  The oracle composition
+    (tile beam, then the station's weighted DFT) runs a 20k-direction sample
+    on the host cores."""
+    import oracle
+    from oskar_amd import lib
+    from oskar_amd.mem import (OSKAR_COMPLEX, OSKAR_GPU, OSKAR_MATRIX,
+                               OSKAR_SINGLE, Mem, Station, StationView,
+                               StationWork, check_status)
+    nt, nd, f = 16, 1_000_000, 150e6
+    g = (np.arange(4) - 1.5) * 1.25
+    tx = np.stack([np.repeat(g, 4), np.tile(g, 4), np.zeros(16)])
+    r = 18.0 * np.sqrt(rng.uniform(size=nt))
+    th = rng.uniform(0, 2 * np.pi, nt)
+    sx = np.stack([r * np.cos(th), r * np.sin(th), np.zeros(nt)])
+    az, el = np.radians(30.0), np.radians(60.0)
+    status = C.c_int(0)
+    st = Station(OSKAR_SINGLE, OSKAR_GPU, nt)
+    st.set_coords(sx)
+    lib.oskar_station_create_child_stations(st.ptr, C.byref(status))
+    for i in range(nt):
+        ch = StationView(lib.oskar_station_child(st.ptr, i), OSKAR_SINGLE)
+        lib.oskar_station_resize(ch.ptr, 16, C.byref(status))
+        ch.set_coords(tx)
+        lib.oskar_station_set_phase_centre(ch.ptr, 4, az, el)
+    lib.oskar_station_set_phase_centre(st.ptr, 4, az, el)
+    lib.oskar_station_analyse(st.ptr, C.byref(C.c_int(0)), C.byref(status))
+    check_status(status, "tiled station set-up")
+    nz = rng.uniform(0, 1, nd)
+    ph = rng.uniform(0, 2 * np.pi, nd)
+    x = np.sqrt(1 - nz * nz) * np.sin(ph)
+    y = np.sqrt(1 - nz * nz) * np.cos(ph)
+    mx, my, mz = (Mem.from_numpy(a, OSKAR_SINGLE) for a in (x, y, nz))
+    out = Mem(OSKAR_SINGLE | OSKAR_COMPLEX | OSKAR_MATRIX, OSKAR_GPU, nd)
+    work = StationWork(OSKAR_SINGLE, OSKAR_GPU)
+
+    def one():
+        lib.oskar_evaluate_station_beam_aperture_array(
+            st.ptr, work.ptr, nd, mx.ptr, my.ptr, mz.ptr, 0, 0.0, f,
+            out.ptr, C.byref(status))
+
+    one()
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    timer = lib.oskar_timer_create(1)
+    lib.oskar_timer_start(timer)
+    for _ in range(args.tile_steps):
+        one()
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    t = lib.oskar_timer_elapsed(timer) / args.tile_steps
+    lib.oskar_timer_free(timer)
+    check_status(status, "tiled station beam")
+    res = {"workload": "16 alike tiles x 16 dipoles, 1M directions, FP32 "
+                       "2x2 beam, 150 MHz",
+           "identical_children": lib.oskar_station_identical_children(
+               st.ptr),
+           "ms": round(1e3 * t, 4), "directions_per_s": nd / t}
+    log("tiled station 16x16 at 1M directions: %.3f ms, %.3e directions/s"
+        % (1e3 * t, nd / t))
+    ns = 20000
+    k = 2 * np.pi * f / C0
+    d = np.array([np.cos(el) * np.sin(az), np.cos(el) * np.cos(az),
+                  np.sin(el)])
+    f4 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    t0 = time.time()
+    tile = oracle.station_beam_aa(
+        1, *[f4(v) for v in tx], *[f4(v) for v in tx], np.zeros(16),
+        np.zeros(16, np.int32), np.array([0], np.int32),
+        np.array([0.5 * C0 / f]), d, f4(x[:ns]), f4(y[:ns]), f4(nz[:ns]), f)
+    w = np.exp(-1j * k * (d @ f4(sx)))
+    oracle.dftw("double", 1, 0, k, np.stack([w.real, w.imag], 1).ravel(),
+                *[f4(v) for v in sx], f4(x[:ns]), f4(y[:ns]), f4(nz[:ns]),
+                None, np.tile(tile, nt))
+    tc = time.time() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res["cpu_baseline"] = {"directions_per_s": ns / tc, "cores": cores,
+                           "kind": "port",
+                           "sample": "the same station, 20k directions"}
+    log("tiled station CPU (oracle, %d cores): %.3e directions/s"
+        % (cores, ns / tc))
+    return res
 
 
 def bench_dftw(args, rng):

@@ -110,3 +110,45 @@ def test_copy_is_recursive():
         assert lib.oskar_station_different(parent.ptr, cp, C.byref(s)) == 1
     finally:
         lib.oskar_station_free(cp, C.byref(C.c_int(0)))
+
+
+def test_feed1_data_host_logic():
+    """Feed-1 element data (oskar_station_set_element_coords.c,
+    oskar_station_accessors.c:259-275, oskar_station_analyse.c:62-79):
+    accessors fall back to feed 0's arrays until feed 1 has its own, which
+    start as a copy of feed 0's; any feed-1 array clears common_pol_beams;
+    copies and comparisons include them."""
+    rng = np.random.default_rng(9)
+    ne = 5
+    xyz = rng.normal(size=(3, ne)) * [[3], [3], [0]]
+    st = Station(OSKAR_DOUBLE, OSKAR_CPU, ne)
+    st.set_coords(xyz)
+    f0 = lib.oskar_station_element_true_enu_metres(st.ptr, 0, 0)
+    assert lib.oskar_station_element_true_enu_metres(st.ptr, 1, 0) == f0
+    assert lib.oskar_station_common_pol_beams(st.ptr) == 1
+    twin = lib.oskar_station_create_copy(st.ptr, OSKAR_CPU, C.byref(C.c_int(0)))
+    s = C.c_int(0)
+    try:
+        lib.oskar_station_set_element_coords(
+            st.ptr, 1, 3, (C.c_double * 3)(1.5, -2.0, 0.0),
+            (C.c_double * 3)(1.5, -2.0, 0.0), C.byref(s))
+        check_status(s, "oskar_station_set_element_coords")
+        f1 = lib.oskar_station_element_true_enu_metres(st.ptr, 1, 0)
+        assert f1 and f1 != f0
+        analyse(st)
+        assert lib.oskar_station_common_pol_beams(st.ptr) == 0
+        from oskar_amd.mem import _Borrowed
+        x1 = _Borrowed(f1).numpy()
+        want = xyz[0].copy()
+        want[3] = 1.5
+        np.testing.assert_array_equal(x1, want)
+        np.testing.assert_array_equal(_Borrowed(f0).numpy(), xyz[0])
+        assert lib.oskar_station_different(st.ptr, twin, C.byref(s)) == 1
+        cp = lib.oskar_station_create_copy(st.ptr, OSKAR_CPU, C.byref(s))
+        assert lib.oskar_station_different(st.ptr, cp, C.byref(s)) == 0
+        lib.oskar_station_free(cp, C.byref(C.c_int(0)))
+        lib.oskar_station_set_element_weight(st.ptr, 2, 0, 1.0, 0.0,
+                                             C.byref(s))
+        assert s.value == -5   # OSKAR_ERR_OUT_OF_RANGE
+    finally:
+        lib.oskar_station_free(twin, C.byref(C.c_int(0)))
