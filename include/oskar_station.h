@@ -140,6 +140,20 @@ OSKAR_EXPORT int oskar_station_apply_element_errors(
         const oskar_Station* station);
 OSKAR_EXPORT int oskar_station_apply_element_weight(
         const oskar_Station* station);
+/* oskar/telescope/station/oskar_station_accessors.h:235-268: the element
+ * gain, gain error, phase offset, phase error and apodisation weight arrays
+ * of a feed (feed 1 shares feed 0's until set for feed 1; NULL until a
+ * value of the field is set). */
+OSKAR_EXPORT const oskar_Mem* oskar_station_element_gain_const(
+        const oskar_Station* station, int feed);
+OSKAR_EXPORT const oskar_Mem* oskar_station_element_gain_error_const(
+        const oskar_Station* station, int feed);
+OSKAR_EXPORT const oskar_Mem* oskar_station_element_phase_offset_rad_const(
+        const oskar_Station* station, int feed);
+OSKAR_EXPORT const oskar_Mem* oskar_station_element_phase_error_rad_const(
+        const oskar_Station* station, int feed);
+OSKAR_EXPORT const oskar_Mem* oskar_station_element_weight_const(
+        const oskar_Station* station, int feed);
 
 /* oskar/telescope/station/oskar_station_set_element_cable_length_error.h */
 OSKAR_EXPORT

@@ -222,7 +222,8 @@ static void resize_zero(oskar_Mem* m, size_t n, int* status)
 
 void oskar_station_resize(oskar_Station* s, int num_elements, int* status)
 {
-    int i = 0, f = 0;
+    int i = 0, f = 0, feed = 0;
+    const int old = s ? s->num_elements : 0;
     if (*status || !s) return;
     for (i = 0; i < 3; ++i)
     {
@@ -235,8 +236,20 @@ void oskar_station_resize(oskar_Station* s, int num_elements, int* status)
             resize_zero(s->element_euler_cpu[f][i], num_elements, status);
     resize_zero(s->element_types, num_elements, status);
     resize_zero(s->element_types_cpu, num_elements, status);
-    for (f = 0; f < F_COUNT; ++f)
-        if (*slot(s, 1, f)) resize_zero(*slot(s, 1, f), num_elements, status);
+    /* Every per-feed array created so far (feed 0's gains, errors, phases
+     * and weights are created on first use, feed 1's arrays all are) takes
+     * the new length; new elements get gain 1 and weight 1 + 0i, the other
+     * fields 0 (oskar_station_resize.c:48-66). */
+    for (feed = 0; feed < 2; ++feed)
+        for (f = feed ? 0 : F_GAIN; f < F_COUNT; ++f)
+        {
+            oskar_Mem* m = *slot(s, feed, f);
+            if (!m) continue;
+            resize_zero(m, (size_t) num_elements, status);
+            if ((f == F_GAIN || f == F_WEIGHT) && num_elements > old)
+                oskar_mem_set_value_real(m, 1.0, (size_t) old,
+                        (size_t) (num_elements - old), status);
+        }
     s->num_elements = num_elements;
 }
 
@@ -632,6 +645,38 @@ int oskar_station_apply_element_errors(const oskar_Station* s)
 int oskar_station_apply_element_weight(const oskar_Station* s)
 {
     return s ? s->apply_element_weight : 0;
+}
+
+/* The arrays a feed's beam uses (feed 1 falls back to feed 0's); NULL until
+ * a value of that field has been set. */
+const oskar_Mem* oskar_station_element_gain_const(const oskar_Station* s,
+        int feed)
+{
+    return s ? feed_mem(s, feed, F_GAIN) : 0;
+}
+
+const oskar_Mem* oskar_station_element_gain_error_const(
+        const oskar_Station* s, int feed)
+{
+    return s ? feed_mem(s, feed, F_GAIN_ERR) : 0;
+}
+
+const oskar_Mem* oskar_station_element_phase_offset_rad_const(
+        const oskar_Station* s, int feed)
+{
+    return s ? feed_mem(s, feed, F_PHASE) : 0;
+}
+
+const oskar_Mem* oskar_station_element_phase_error_rad_const(
+        const oskar_Station* s, int feed)
+{
+    return s ? feed_mem(s, feed, F_PHASE_ERR) : 0;
+}
+
+const oskar_Mem* oskar_station_element_weight_const(const oskar_Station* s,
+        int feed)
+{
+    return s ? feed_mem(s, feed, F_WEIGHT) : 0;
 }
 
 void oskar_station_set_element_feed_angle(oskar_Station* s, int feed,

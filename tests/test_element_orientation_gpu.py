@@ -110,3 +110,60 @@ def test_element_type_out_of_range(hip):
     lib.oskar_station_set_phase_centre(st.ptr, AZEL, 0.0, 1.2)
     x, y, z = directions(rng, 64)
     beam("d", st, x, y, z, expect_status=-5)  # OSKAR_ERR_OUT_OF_RANGE
+
+
+@pytest.mark.parametrize("prec,swap_xy", [("d", 0), ("d", 1), ("f", 0)])
+def test_mixed_angles_closed_form(hip, prec, swap_xy):
+    """Every element with its own X and Y feed angles: the beam equals
+    sum_a exp(-i k xm_a.b) exp(i k xt_a.s) g_a(s), with g_a the textbook
+    dipole field of that element's two feeds (X along (cos ax, sin ax, 0),
+    Y along (sin ay, -cos ay, 0)); no oracle code is involved."""
+    from dipole_closed_form import dipole_matrix
+    print("per-element orientation closed form", prec, swap_xy, flush=True)
+    rng = np.random.default_rng(63)
+    ne = 40
+    r = 15.0 * np.sqrt(rng.uniform(size=ne))
+    t = rng.uniform(0, 2 * np.pi, ne)
+    xt = np.stack([r * np.cos(t), r * np.sin(t), np.zeros(ne)])
+    xm = xt + np.concatenate([rng.normal(scale=0.01, size=(2, ne)),
+                              np.zeros((1, ne))])
+    ax = rng.uniform(-180, 180, ne)
+    ay = ax + rng.uniform(-20, 20, ne)
+    P = OSKAR_DOUBLE if prec == "d" else OSKAR_SINGLE
+    st = Station(P, OSKAR_GPU, ne)
+    st.set_coords(xt, xm)
+    s = C.c_int(0)
+    for i in range(ne):
+        lib.oskar_station_set_element_feed_angle(
+            st.ptr, 0, i, float(ax[i]), 0.0, 0.0, C.byref(s))
+        lib.oskar_station_set_element_feed_angle(
+            st.ptr, 1, i, float(ay[i]), 0.0, 0.0, C.byref(s))
+    check_status(s, "oskar_station_set_element_feed_angle")
+    lib.oskar_station_set_swap_xy(st.ptr, swap_xy)
+    analyse(st)
+    assert lib.oskar_station_common_element_orientation(st.ptr) == 0
+    pointing = (math.radians(-40), math.radians(55))
+    lib.oskar_station_set_phase_centre(st.ptr, AZEL, *pointing)
+    x, y, z = directions(rng, 800)
+    got = beam(prec, st, x, y, z).reshape(-1, 4, 2)
+    got = (got[..., 0] + 1j * got[..., 1]).reshape(-1, 2, 2)
+    if prec == "f":
+        f = lambda a: np.asarray(a, np.float32).astype(np.float64)  # noqa
+        xt, xm, x, y, z = f(xt), f(xm), f(x), f(y), f(z)
+    k = 2 * np.pi * FREQ / C0
+    b = azel_dir(*pointing)
+    w = np.exp(-1j * k * (xm.T @ b))
+    want = np.zeros((x.size, 2, 2), complex)
+    for a in range(ne):
+        ph = np.exp(1j * k * (xt[0, a] * x + xt[1, a] * y + xt[2, a] * z))
+        want += (w[a] * ph)[:, None, None] * dipole_matrix(
+            math.radians(ax[a]), math.radians(ay[a]), x, y, z, np.pi / 2,
+            swap_xy)
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    rms = rel_rms(np.stack([got.real, got.imag]),
+                  np.stack([want.real, want.imag]))
+    print("max err / peak %.3e, rel RMS %.3e" % (err, rms), flush=True)
+    if prec == "d":
+        assert err < 1e-10
+    else:
+        assert rms < 2e-5

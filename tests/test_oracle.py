@@ -344,3 +344,89 @@ def test_factorised_reference_matches_direct_oracle():
     fact = oracle.factorised_subset(l, m, n, I_, E.ravel(), u, v, w, k,
                                     np.array(p), np.array(q))
     assert rel_rms(fact, direct) < 1e-13
+
+
+def _single_dipole_beam(ex, ey, x, y, z, freq, swap_xy=0):
+    """The oracle station beam of one dipole at the origin (array factor 1,
+    no normalisation): the element pattern alone, (n, 2, 2) complex."""
+    zero = np.zeros(1)
+    beam = oracle.station_beam_aa(
+        1, zero, zero, zero, zero, zero, zero, zero, np.zeros(1, np.int32),
+        np.array([0], np.int32), np.array([0.5 * C0 / freq]),
+        np.array([0.0, 0.0, 1.0]), x, y, z, freq, euler_x0=ex, euler_y0=ey,
+        swap_xy=swap_xy)
+    b = beam.reshape(-1, 4, 2)
+    return (b[..., 0] + 1j * b[..., 1]).reshape(-1, 2, 2)
+
+
+@pytest.mark.parametrize("ex_deg,ey_deg,swap_xy", [
+    (0.0, 0.0, 0), (90.0, 90.0, 0), (30.0, 30.0, 0), (-50.0, -50.0, 1),
+    (20.0, 75.0, 0)])
+def test_station_beam_oracle_dipole_axes_closed_form(ex_deg, ey_deg, swap_xy):
+    """The dipole branch of the AA-beam oracle (feed offsets from the Euler
+    angles, theta/phi, dipole pattern, row order) equals the textbook dipole
+    field in vector form: X along (cos ex, sin ex, 0), Y along
+    (sin ey, -cos ey, 0).  The X feed is null towards its own axis on the
+    horizon and maximal across it; the Y feed likewise."""
+    from dipole_closed_form import dipole_matrix, feed_axes
+    freq = 120e6
+    kh = np.pi / 2                      # half-wave dipole: h = lambda/4
+    ex, ey = np.radians(ex_deg), np.radians(ey_deg)
+    rng = np.random.default_rng(77)
+    n = 400
+    th = np.arccos(rng.uniform(0.0, 1.0, n))
+    ph = rng.uniform(0, 2 * np.pi, n)
+    dx, dy = feed_axes(ex, ey)
+    horizon = np.stack([dx, -dx, dy, -dy])
+    x = np.concatenate([np.sin(th) * np.cos(ph), horizon[:, 0]])
+    y = np.concatenate([np.sin(th) * np.sin(ph), horizon[:, 1]])
+    z = np.concatenate([np.cos(th), horizon[:, 2]])
+    got = _single_dipole_beam(ex, ey, x, y, z, freq, swap_xy)
+    want = dipole_matrix(ex, ey, x, y, z, kh, swap_xy)
+    assert np.max(np.abs(got - want)) < 1e-12
+    rx, ry = (1, 0) if swap_xy else (0, 1)
+    # Nulls along each feed's own axis; the other feed is broadside there
+    # (|E| = 1 - cos(kh) = 1 at kh = pi/2).
+    hx, hy = got[n:n + 2], got[n + 2:n + 4]
+    assert np.all(np.abs(hx[:, rx, :]) < 1e-7)
+    assert np.all(np.abs(hy[:, ry, :]) < 1e-7)
+    if abs(ex - ey) < 1e-12:
+        assert np.allclose(np.linalg.norm(hx[:, ry, :], axis=1), 1.0)
+        assert np.allclose(np.linalg.norm(hy[:, rx, :], axis=1), 1.0)
+    # The check separates the plausible mistakes: X and Y axes swapped, the
+    # Euler angle's sign flipped, the field's sign flipped.
+    wrong = [dipole_matrix(ey, ex, x, y, z, kh, 1 - swap_xy),
+             -want]
+    if ex != 0.0 or ey != 0.0:
+        wrong.append(dipole_matrix(-ex, -ey, x, y, z, kh, swap_xy))
+    for w in wrong:
+        assert np.max(np.abs(got - w)) > 0.1
+
+
+def test_station_beam_oracle_dipole_array_closed_form():
+    """Dipoles at random positions with a common orientation: the beam is
+    sum_a exp(i k x_a.(s - b)) g(s) with g the textbook dipole field."""
+    from dipole_closed_form import dipole_matrix
+    freq = 150e6
+    k = 2 * np.pi * freq / C0
+    rng = np.random.default_rng(78)
+    ne = 24
+    r = 12 * np.sqrt(rng.uniform(size=ne))
+    t = rng.uniform(0, 2 * np.pi, ne)
+    xe, ye, ze = r * np.cos(t), r * np.sin(t), np.zeros(ne)
+    b = np.array([0.3, -0.2, np.sqrt(1 - 0.13)])
+    n = 300
+    th = np.arccos(rng.uniform(0.0, 1.0, n))
+    ph = rng.uniform(0, 2 * np.pi, n)
+    x, y, z = np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)
+    ex = np.radians(40.0)
+    beam = oracle.station_beam_aa(
+        1, xe, ye, ze, xe, ye, ze, np.zeros(ne), np.zeros(ne, np.int32),
+        np.array([0], np.int32), np.array([0.5 * C0 / freq]), b, x, y, z,
+        freq, euler_x0=ex, euler_y0=ex)
+    bb = beam.reshape(-1, 4, 2)
+    got = (bb[..., 0] + 1j * bb[..., 1]).reshape(-1, 2, 2)
+    af = np.exp(1j * k * (np.outer(x, xe) + np.outer(y, ye)
+                          - (xe * b[0] + ye * b[1]))).sum(1)
+    want = af[:, None, None] * dipole_matrix(ex, ex, x, y, z, np.pi / 2)
+    assert np.max(np.abs(got - want)) < 1e-10 * ne
