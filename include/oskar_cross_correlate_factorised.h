@@ -59,6 +59,50 @@ void oskar_cross_correlate_factorised(
         oskar_Mem* vis,
         int* status);
 
+/*
+ * One station beam per station (no counterpart in the reference; the case
+ * oskar_evaluate_jones_E produces when station beams are not duplicated,
+ * oskar/interferometer/src/oskar_evaluate_jones_E.c:126-140):
+ *
+ *   V_pq = sum_s K_p(s) conj(K_q(s)) E_p(s) B_s E_q(s)^H,
+ *
+ * with E an oskar_Jones of num_stations x (>= num_sources) complex 2x2
+ * matrices in the precision of vis.  Each B_s is factorised as
+ * L_s D_s L_s^H (D = identity for any source with |pol| <= I; +-1 signs
+ * otherwise, so every B is allowed), A_p(s) = K_p(s) E_p(s) L_s, and
+ * V = A D A^H is one Hermitian rank-2S product on split-16-bit MFMA with
+ * FP32 accumulation per part of sources and FP64 sums over parts.  The
+ * station phases and beams enter once per (station, source).  Other
+ * arguments, checks, accumulation and errors as
+ * oskar_cross_correlate_factorised(); the Jones must be on the device of
+ * vis.  Relative RMS error vs FP64 about 1e-6 (FP16X3).
+ */
+OSKAR_EXPORT
+void oskar_cross_correlate_factorised_jones(
+        int use_casa_phase_convention,
+        int num_sources,
+        const oskar_Jones* E,
+        const oskar_Mem* const src_flux[4],
+        const oskar_Mem* const src_dir[3],
+        const oskar_Telescope* tel,
+        const oskar_Mem* const station_uvw[3],
+        double frequency_hz,
+        int mode,
+        int offset_out,
+        oskar_Mem* vis,
+        int* status);
+
+/* Sources per part of oskar_cross_correlate_factorised_jones() (default
+ * and maximum 2048; multiple of 4).  Returns the previous value. */
+OSKAR_EXPORT int oskar_cross_correlate_factorised_jones_set_part_size(
+        int sources);
+
+/* As oskar_cross_correlate_factorised_last_times() for the last profiled
+ * oskar_cross_correlate_factorised_jones() call: ms[0..2] = operands
+ * (sources, phases, A = K E L, split), MFMA product, reduction. */
+OSKAR_EXPORT int oskar_cross_correlate_factorised_jones_last_times(
+        double ms[3]);
+
 /* Sources accumulated in FP32 by one workgroup before the FP64 sum
  * (default 8192; multiple of 8).  Returns the previous value. */
 OSKAR_EXPORT int oskar_cross_correlate_factorised_set_part_size(int sources);
@@ -69,6 +113,14 @@ OSKAR_EXPORT int oskar_cross_correlate_factorised_set_part_size(int sources);
  * Returns the previous shape; a negative shape only returns the current
  * one; a shape past the last returns -1 and changes nothing. */
 OSKAR_EXPORT int oskar_cross_correlate_factorised_set_tile_shape(int shape);
+
+/* Station phasors K_p(s) of oskar_cross_correlate_factorised(): 1 (default)
+ * computes each (station, source) phasor once per call into device planes
+ * (8 bytes per station and source) that the tiles read; 0 computes them
+ * inside every tile that needs them (no planes; the wave-specialised shape
+ * 3 always does).  Same arithmetic, same results.  Returns the previous
+ * setting; a negative value only returns it. */
+OSKAR_EXPORT int oskar_cross_correlate_factorised_set_phasor_planes(int on);
 
 /* Stage timing with HIP events on the launch stream (off by default).
  * last_times() waits for the last profiled call on the current device and

@@ -78,6 +78,23 @@ OSKAR_EXPORT oskar_Station* oskar_telescope_station(oskar_Telescope* model,
         int i);
 OSKAR_EXPORT const oskar_Station* oskar_telescope_station_const(
         const oskar_Telescope* model, int i);
+/* Station beam duplication (oskar/telescope/src/oskar_telescope_accessors.c
+ * :137-140, 187-196, 300-304, 609-621): with duplication allowed, station i
+ * uses model type_map[i] and stations of one model share one evaluated
+ * beam; otherwise station i uses model i.  The type map (host int array of
+ * num_stations entries) starts at 0 for every station;
+ * oskar_telescope_set_unique_stations(1) sets type_map[i] = i, (0) clears
+ * it. */
+OSKAR_EXPORT int oskar_telescope_allow_station_beam_duplication(
+        const oskar_Telescope* model);
+OSKAR_EXPORT void oskar_telescope_set_allow_station_beam_duplication(
+        oskar_Telescope* model, int value);
+OSKAR_EXPORT oskar_Mem* oskar_telescope_station_type_map(
+        oskar_Telescope* model);
+OSKAR_EXPORT const oskar_Mem* oskar_telescope_station_type_map_const(
+        const oskar_Telescope* model);
+OSKAR_EXPORT void oskar_telescope_set_unique_stations(oskar_Telescope* model,
+        int value, int* status);
 /* units: "Metres" or "Wavelengths" (case-insensitive on the first letter),
  * oskar/telescope/src/oskar_telescope_accessors.c. */
 OSKAR_EXPORT void oskar_telescope_set_uv_filter(oskar_Telescope* model,

@@ -19,7 +19,18 @@ int oskar_amd_fact_launch(int precision, int mode, int use_casa,
         double uv_min_lambda, double uv_max_lambda, int offset_out,
         void* vis, int src_per_part, void* stream);
 
+int oskar_amd_fact_stations_launch(int precision, int mode, int use_casa,
+        int num_sources, int num_stations, const void* E, size_t ld_e,
+        const void* I, const void* Q, const void* U, const void* V,
+        const void* l, const void* m, const void* n, const void* st_u,
+        const void* st_v, const void* st_w, double frequency_hz,
+        double uv_min_lambda, double uv_max_lambda, int offset_out,
+        void* vis, int src_per_part, void* stream);
+void oskar_amd_fact_stations_set_profiling(int on);
+int oskar_amd_fact_stations_last_times(double* ms);
+
 void oskar_amd_fact_set_profiling(int on);
+int oskar_amd_fact_set_phasor_planes(int on);
 int oskar_amd_fact_set_tile_shape(int shape);
 int oskar_amd_fact_last_times(double* ms);
 
@@ -42,10 +53,17 @@ int oskar_amd_fact_batch_autotune(void* b, int num_units, int reps,
         void* stream, double* best_ms);
 
 static int part_size = 8192;
+static int part_size_jones = 2048;
 
 void oskar_cross_correlate_factorised_set_profiling(int on)
 {
     oskar_amd_fact_set_profiling(on);
+    oskar_amd_fact_stations_set_profiling(on);
+}
+
+int oskar_cross_correlate_factorised_jones_last_times(double ms[3])
+{
+    return oskar_amd_fact_stations_last_times(ms);
 }
 
 int oskar_cross_correlate_factorised_last_times(double ms[3])
@@ -58,10 +76,23 @@ int oskar_cross_correlate_factorised_set_tile_shape(int shape)
     return oskar_amd_fact_set_tile_shape(shape);
 }
 
+int oskar_cross_correlate_factorised_set_phasor_planes(int on)
+{
+    return oskar_amd_fact_set_phasor_planes(on);
+}
+
 int oskar_cross_correlate_factorised_set_part_size(int sources)
 {
     const int old = part_size;
     if (sources > 0) part_size = ((sources + 7) / 8) * 8;
+    return old;
+}
+
+int oskar_cross_correlate_factorised_jones_set_part_size(int sources)
+{
+    const int old = part_size_jones;
+    if (sources > 0)
+        part_size_jones = sources > 2048 ? 2048 : ((sources + 3) / 4) * 4;
     return old;
 }
 
@@ -201,6 +232,71 @@ void oskar_cross_correlate_factorised(int use_casa, int num_sources,
                 oskar_mem_void_const(station_uvw[2]), frequency_hz,
                 uv_min, uv_max, offset_out, oskar_mem_void(vis), part_size,
                 oskar_amd_current_stream());
+        if (e) *status = e;
+    }
+}
+
+void oskar_cross_correlate_factorised_jones(int use_casa, int num_sources,
+        const oskar_Jones* E, const oskar_Mem* const src_flux[4],
+        const oskar_Mem* const src_dir[3], const oskar_Telescope* tel,
+        const oskar_Mem* const station_uvw[3], double frequency_hz,
+        int mode, int offset_out, oskar_Mem* vis, int* status)
+{
+    double uv_min = 0.0, uv_max = 0.0;
+    const oskar_Mem* e_mem = 0;
+    if (*status) return;
+    if (!E || !vis || !tel)
+    {
+        *status = OSKAR_ERR_INVALID_ARGUMENT;
+        return;
+    }
+    const int num_stations = oskar_telescope_num_stations(tel);
+    frequency_hz = fabs(frequency_hz);
+    uv_filter(tel, frequency_hz, &uv_min, &uv_max);
+    const int location = oskar_mem_location(vis);
+    const int prec = oskar_mem_precision(vis);
+    e_mem = oskar_jones_mem_const(E);
+    if (oskar_telescope_mem_location(tel) != location ||
+            oskar_jones_mem_location(E) != location)
+    {
+        *status = OSKAR_ERR_LOCATION_MISMATCH;
+        return;
+    }
+    *status = check_unit(location, num_stations, num_sources, e_mem,
+            src_flux, src_dir, station_uvw, offset_out, vis);
+    if (*status) return;
+    if (oskar_jones_num_stations(E) != num_stations ||
+            oskar_jones_num_sources(E) < num_sources)
+    {
+        *status = OSKAR_ERR_DIMENSION_MISMATCH;
+        return;
+    }
+    if (mode != OSKAR_FACTORISED_FP16X3 && mode != OSKAR_FACTORISED_BF16X3)
+    {
+        *status = OSKAR_ERR_INVALID_ARGUMENT;
+        return;
+    }
+    if (smearing_on(tel))
+    {
+        *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE;
+        return;
+    }
+    {
+        const int e = oskar_amd_fact_stations_launch(prec, mode, use_casa,
+                num_sources, num_stations, oskar_mem_void_const(e_mem),
+                (size_t) oskar_jones_num_sources(E),
+                oskar_mem_void_const(src_flux[0]),
+                oskar_mem_void_const(src_flux[1]),
+                oskar_mem_void_const(src_flux[2]),
+                oskar_mem_void_const(src_flux[3]),
+                oskar_mem_void_const(src_dir[0]),
+                oskar_mem_void_const(src_dir[1]),
+                oskar_mem_void_const(src_dir[2]),
+                oskar_mem_void_const(station_uvw[0]),
+                oskar_mem_void_const(station_uvw[1]),
+                oskar_mem_void_const(station_uvw[2]), frequency_hz,
+                uv_min, uv_max, offset_out, oskar_mem_void(vis),
+                part_size_jones, oskar_amd_current_stream());
         if (e) *status = e;
     }
 }

@@ -226,7 +226,40 @@ oskar_Telescope* oskar_telescope_create(int type, int location,
     for (i = 0; i < 3; ++i)
         t->station_true_offset_ecef_metres[i] =
                 oskar_mem_create(type, location, (size_t) num_stations, status);
+    /* Every station of type 0 (oskar_telescope_create.c:56-58). */
+    t->station_type_map = oskar_mem_create(OSKAR_INT, OSKAR_CPU,
+            (size_t) num_stations, status);
+    oskar_mem_clear_contents(t->station_type_map, status);
     return t;
+}
+
+/* oskar/telescope/src/oskar_telescope_accessors.c:137-140, 187-196,
+ * 300-304, 609-621. */
+int oskar_telescope_allow_station_beam_duplication(const oskar_Telescope* t)
+{ return t->allow_station_beam_duplication; }
+
+void oskar_telescope_set_allow_station_beam_duplication(oskar_Telescope* t,
+        int value)
+{ t->allow_station_beam_duplication = value; }
+
+oskar_Mem* oskar_telescope_station_type_map(oskar_Telescope* t)
+{ return t->station_type_map; }
+
+const oskar_Mem* oskar_telescope_station_type_map_const(
+        const oskar_Telescope* t)
+{ return t->station_type_map; }
+
+void oskar_telescope_set_unique_stations(oskar_Telescope* t, int value,
+        int* status)
+{
+    int i = 0;
+    if (*status) return;
+    if (value)
+    {
+        int* map = (int*) oskar_mem_void(t->station_type_map);
+        for (i = 0; i < t->num_stations; ++i) map[i] = i;
+    }
+    else oskar_mem_clear_contents(t->station_type_map, status);
 }
 
 void oskar_telescope_free(oskar_Telescope* t, int* status)
@@ -235,6 +268,7 @@ void oskar_telescope_free(oskar_Telescope* t, int* status)
     if (!t) return;
     for (i = 0; i < 3; ++i)
         oskar_mem_free(t->station_true_offset_ecef_metres[i], status);
+    oskar_mem_free(t->station_type_map, status);
     for (i = 0; i < t->num_station_models; ++i)
         oskar_station_free(t->station[i], status);
     free(t->station);

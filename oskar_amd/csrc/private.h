@@ -58,6 +58,8 @@ struct oskar_Telescope
     oskar_Mem* station_true_offset_ecef_metres[3];
     int num_station_models;      /* private_telescope.h:54 */
     oskar_Station** station;     /* station models */
+    oskar_Mem* station_type_map; /* CPU int, num_stations (private_telescope.h:61) */
+    int allow_station_beam_duplication;
 };
 
 /* oskar/telescope/station/element/private_element.h (dipole/isotropic). */

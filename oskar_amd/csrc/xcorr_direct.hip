@@ -36,6 +36,7 @@
 
 #include "device_common.h"
 #include "private.h"
+#include "workspace.h"
 
 using namespace oskar_amd;
 
@@ -401,26 +402,14 @@ void xcorr_reduce_kernel(XcorrArgs<FP> A)
     for (int r = 0; r < NR; ++r) v[r] += (FP) sum[r];
 }
 
-/* Per-device FP64 workspace for the split partial sums, grown on demand and
- * kept for the life of the process. */
-void* workspace_get(size_t bytes, hipError_t* err)
+/* FP64 workspace for the split partial sums, one per (device, stream),
+ * grown on demand and kept for the life of the process. */
+void* workspace_get(size_t bytes, hipStream_t stream, hipError_t* err)
 {
-    static void* ptr[64];
-    static size_t size[64];
     int dev = 0;
     *err = hipGetDevice(&dev);
     if (*err != hipSuccess) return nullptr;
-    if (dev < 0 || dev >= 64) { *err = hipErrorInvalidDevice; return nullptr; }
-    if (size[dev] < bytes)
-    {
-        if (ptr[dev]) (void) hipFree(ptr[dev]);
-        ptr[dev] = nullptr;
-        size[dev] = 0;
-        *err = hipMalloc(&ptr[dev], bytes);
-        if (*err != hipSuccess) { ptr[dev] = nullptr; return nullptr; }
-        size[dev] = bytes;
-    }
-    return ptr[dev];
+    return ws_acquire(WS_XCORR_SPLIT, dev, stream, bytes, err);
 }
 
 template <typename FP, bool MATRIX, bool GAUSS, bool BS, bool TS>
@@ -458,7 +447,7 @@ hipError_t launch_t(XcorrArgs<FP> A, hipStream_t stream)
     {
         hipError_t err = hipSuccess;
         A.partial = (double*) workspace_get(
-                (size_t) splits * nb * G::NR * sizeof(double), &err);
+                (size_t) splits * nb * G::NR * sizeof(double), stream, &err);
         if (err != hipSuccess) return err;
     }
     dim3 grid(na, nbk, splits);

@@ -69,3 +69,31 @@ def oracle_direct(c, freq, casa=0, **kw):
         "double", casa, 0, 1, jr, c["I"], c["Q"], c["U"], c["V"], c["l"],
         c["m"], c["n"], None, None, None, c["u"], c["v"], c["w"], z, z,
         freq=freq, **kw)
+
+
+def station_beams(seed, ns, nsrc, spread=0.3):
+    """One 2x2 complex beam per (station, source): identity plus complex
+    Gaussian noise, (ns, nsrc, 8) reals."""
+    rng = np.random.default_rng(seed)
+    E = rng.normal(size=(ns, nsrc, 8)) * spread
+    E[..., 0] += 1.0
+    E[..., 6] += 1.0
+    return E
+
+
+def oracle_direct_stations(c, E, freq, casa=0, **kw):
+    """The FP64 oracle of the reference's direct algorithm on
+    J_p(s) = K_p(s) E_p(s) with one beam per station."""
+    ns, nsrc = c["u"].size, c["l"].size
+    k = 2 * np.pi * freq / C0
+    K = oracle.jones_K(c["l"], c["m"], c["n"], c["u"], c["v"], c["w"], k,
+                       np.ones(nsrc)).reshape(ns, nsrc, 2)
+    Kc = K[..., 0] + 1j * K[..., 1]
+    Ec = (E[..., 0::2] + 1j * E[..., 1::2]).reshape(ns, nsrc, 2, 2)
+    J = Kc[:, :, None, None] * Ec
+    jr = np.stack([J.real, J.imag], -1).ravel()
+    z = np.zeros(ns)
+    return oracle.cross_correlate(
+        "double", casa, 0, 1, jr, c["I"], c["Q"], c["U"], c["V"], c["l"],
+        c["m"], c["n"], None, None, None, c["u"], c["v"], c["w"], z, z,
+        freq=freq, **kw)

@@ -127,3 +127,41 @@ def gpu_factorised(prec, mode, casa, beam, I_, Q, U, V, l, m, n, u, v, w,
     check_status(s, "oskar_cross_correlate_factorised")
     sync_check("oskar_cross_correlate_factorised")
     return vis.numpy()
+
+
+def gpu_factorised_jones(prec, mode, casa, E, I_, Q, U, V, l, m, n, u, v, w,
+                         freq, vis_init=None, offset_out=0, uv_filter=None,
+                         num_sources=None, expect_status=0):
+    """oskar_cross_correlate_factorised_jones on the GPU with one beam per
+    station, E (stations, sources, 8 reals); returns vis reals."""
+    P = prec_code(prec)
+    ns = u.size
+    nsrc = l.size if num_sources is None else num_sources
+    mt = P | OSKAR_COMPLEX | OSKAR_MATRIX
+    tel = Telescope(P, OSKAR_GPU, ns)
+    if uv_filter is not None:
+        s = C.c_int(0)
+        lib.oskar_telescope_set_uv_filter(tel.ptr, uv_filter[0], uv_filter[1],
+                                          b"Metres", C.byref(s))
+        check_status(s, "set_uv_filter")
+    flux = [Mem.from_numpy(a, P) for a in (I_, Q, U, V)]
+    dirs = [Mem.from_numpy(a, P) for a in (l, m, n)]
+    uvw = [Mem.from_numpy(a, P) for a in (u, v, w)]
+    jones = Jones(mt, OSKAR_GPU, E.shape[0], E.shape[1])
+    jones.set(np.ascontiguousarray(E).ravel())
+    nb = ns * (ns - 1) // 2
+    if vis_init is None:
+        vis_init = np.zeros((offset_out + nb) * 8)
+    vis = Mem.from_numpy(vis_init, mt)
+    s = C.c_int(0)
+    lib.oskar_cross_correlate_factorised_jones(
+        int(casa), int(nsrc), jones.ptr, capi.MEMA(*[x.ptr for x in flux]),
+        capi.MEM3(*[x.ptr for x in dirs]), tel.ptr,
+        capi.MEM3(*[x.ptr for x in uvw]), float(freq), int(mode),
+        int(offset_out), vis.ptr, C.byref(s))
+    if expect_status:
+        assert s.value == expect_status, s.value
+        return None
+    check_status(s, "oskar_cross_correlate_factorised_jones")
+    sync_check("oskar_cross_correlate_factorised_jones")
+    return vis.numpy()

@@ -1,0 +1,122 @@
+"""The learned beam surrogate at its own workload, against the method
+without it (the exact station beam,
+oskar_evaluate_station_beam_aperture_array.c:207-224): the 256-dipole
+station (elements uniform in a 38 m disc, beam 30 deg from zenith) over
+50-350 MHz, trained on the GPU on 2M exact-beam samples, evaluated at
+150 MHz on 1M directions above the horizon.
+
+Reported: the beam error against the exact beam, the directions below
+1e-3 of the peak that fall back to the exact beam, and the time of the
+fused inference kernel against the exact beam of this station and of
+larger stations on the same directions.
+
+Asserted (the trade-off as measured): training reduces the beam error by
+an order of magnitude against the untrained network, to below 10 % relative
+RMS; every fallback direction carries the exact beam bit for bit; the
+surrogate's time does not depend on the element count, so it is slower
+than the exact beam at 256 elements and faster than it at 16,384 elements
+(the exact cost grows linearly with elements).  At 4 % beam error the
+surrogate is far from the 1e-5 visibility target: the exact beam, not the
+surrogate, feeds the headline pipeline.
+"""
+import ctypes as C
+import math
+import time
+
+import numpy as np
+import pytest
+
+from gpu_util import sync_check
+from oskar_amd import lib
+from oskar_amd.mem import (OSKAR_COMPLEX, OSKAR_GPU, OSKAR_MATRIX,
+                           OSKAR_SINGLE, BeamSurrogate, Mem, Station,
+                           StationWork, beam_training_set, check_status)
+from workloads import elements, sky
+
+pytestmark = pytest.mark.gpu
+CM32 = OSKAR_SINGLE | OSKAR_COMPLEX | OSKAR_MATRIX
+FREQ = 150e6
+
+
+def make(ne, radius, rng):
+    st = Station(OSKAR_SINGLE, OSKAR_GPU, ne)
+    st.set_coords(elements(rng, ne, radius))
+    lib.oskar_station_set_phase_centre(st.ptr, 4, 0.0, math.radians(60.0))
+    return st
+
+
+def timed_ms(fn, reps=10):
+    fn()
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    lib.oskar_device_synchronize(OSKAR_GPU)
+    return 1e3 * (time.perf_counter() - t0) / reps
+
+
+def test_config5_surrogate_against_exact_beam(hip):
+    rng = np.random.default_rng(505)
+    station = make(256, 19.0, rng)
+    work = StationWork(OSKAR_SINGLE, OSKAR_GPU)
+    l0, m0 = 0.0, 0.5
+    tl, tm, tf, tb = beam_training_set(station, work, 40, 50000, 50e6,
+                                       350e6, l0, m0, seed=3)
+    sur = BeamSurrogate(50e6, 350e6, l0, m0, 19.0, seed=4)
+    S = 1_000_000
+    sk = sky(rng, S)
+    dirs = [Mem.from_numpy(sk[k], OSKAR_SINGLE) for k in ("l", "m", "n")]
+    exact = Mem(CM32, OSKAR_GPU, S)
+    out = Mem(CM32, OSKAR_GPU, S)
+    s = C.c_int(0)
+
+    def run_exact(st=station, w=work, o=exact):
+        lib.oskar_evaluate_station_beam_aperture_array(
+            st.ptr, w.ptr, S, dirs[0].ptr, dirs[1].ptr, dirs[2].ptr, 0,
+            0.0, FREQ, o.ptr, C.byref(s))
+
+    def run_sur():
+        sur.evaluate(S, dirs[0], dirs[1], FREQ, out)
+
+    def beam_err():
+        run_exact()
+        run_sur()
+        sync_check("beams")
+        e = exact.numpy().astype(np.float64)
+        g = out.numpy().astype(np.float64)
+        return float(np.sqrt(np.sum((g - e) ** 2) / np.sum(e * e)))
+
+    err0 = beam_err()
+    t0 = time.time()
+    hist = sur.train(len(tl), tl, tm, tf, tb, 8192, 6000, 2e-3, 2e-5, seed=6)
+    sync_check("train")
+    t_train = time.time() - t0
+    err1 = beam_err()
+    nfb = sur.evaluate_with_fallback(station, work, S, dirs[0], dirs[1],
+                                     dirs[2], FREQ, 1e-3, out)
+    sync_check("fallback")
+    got = out.numpy().reshape(S, 8)
+    ex = exact.numpy().reshape(S, 8)
+    n_exact = int(np.sum(np.all(got == ex, axis=1)))
+    t_sur = timed_ms(run_sur)
+    t_256 = timed_ms(run_exact)
+    times = {256: t_256}
+    for ne, radius in ((4096, 76.0), (16384, 152.0)):
+        big = make(ne, radius, rng)
+        wb = StationWork(OSKAR_SINGLE, OSKAR_GPU)
+        ob = Mem(CM32, OSKAR_GPU, S)
+        times[ne] = timed_ms(lambda: run_exact(big, wb, ob), reps=3)
+    check_status(s, "exact beams")
+    print("config 5: %d training samples, 6000 steps in %.1f s (loss %.2e "
+          "-> %.2e); beam rel RMS vs exact at 150 MHz: untrained %.3e, "
+          "trained %.3e; %d of 1M directions below 1e-3 of the peak take "
+          "the exact beam (%d rows equal the exact beam); inference %.3f "
+          "ms per 1M directions; exact beam %s ms" % (
+              len(tl), t_train, hist[:10].mean(), hist[-50:].mean(), err0,
+              err1, nfb, n_exact,
+              t_sur, {k: round(v, 3) for k, v in times.items()}),
+          flush=True)
+    assert err1 < 0.1 * err0 and err1 < 0.1
+    assert nfb > 0 and n_exact >= nfb
+    assert times[16384] > 8 * times[256]
+    assert t_256 < t_sur < times[16384]
