@@ -153,6 +153,9 @@ struct oskar_StationWork
      * beam[], num_depths); depth 0 is the element-pattern signal above. */
     oskar_Mem** depth_beam;
     int num_depths;
+    /* int zeros, one per child of a station whose children are alike: the
+     * data index that makes every child read the one evaluated beam. */
+    oskar_Mem* zero_idx;
 };
 
 /* oskar/sky/private_sky.h: one table of num_columns x capacity values

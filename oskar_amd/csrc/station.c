@@ -18,6 +18,7 @@
  *  - oskar/convert/define_convert_apparent_ra_dec_to_enu_directions.h.
  */
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -964,6 +965,7 @@ void oskar_station_work_free(oskar_StationWork* w, int* status)
     oskar_mem_free(w->beam_out, status);
     oskar_mem_free(w->horizon_mask, status);
     oskar_mem_free(w->source_indices, status);
+    oskar_mem_free(w->zero_idx, status);
     {
         int i = 0;
         for (i = 0; i < w->num_depths; ++i)
@@ -1362,6 +1364,21 @@ static oskar_Mem* work_beam(oskar_StationWork* w, const oskar_Mem* beam,
     return *m;
 }
 
+/* At least n int zeros in the work struct (cleared when grown). */
+static const oskar_Mem* work_zero_idx(oskar_StationWork* w, int n,
+        int* status)
+{
+    if (*status) return 0;
+    if (!w->zero_idx || (int) oskar_mem_length(w->zero_idx) < n)
+    {
+        oskar_mem_free(w->zero_idx, status);
+        w->zero_idx = oskar_mem_create(OSKAR_INT, OSKAR_GPU, (size_t) n,
+                status);
+        oskar_mem_clear_contents(w->zero_idx, status);
+    }
+    return w->zero_idx;
+}
+
 /* Element weights of one station for the beam direction, with its element
  * errors and apodisation (oskar_station_evaluate_element_weights.c:15-66). */
 static void station_weights(const oskar_Station* s, int feed,
@@ -1442,20 +1459,24 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
             *status = OSKAR_ERR_SETTINGS_TELESCOPE;
             return;
         }
-        signal = work_beam(work, beam,
-                (size_t) s->num_elements * (size_t) num_points, depth, status);
+        const oskar_Mem* child_idx = 0;
         if (s->identical_children && s->num_elements > 0)
         {
+            /* One evaluated child beam serves every element through an
+             * all-zero data index (the reference copies it to each
+             * element's row, :253-262; the DFT reads the same values). */
+            signal = work_beam(work, beam, (size_t) num_points, depth,
+                    status);
             aa_beam(s->child[0], work, offset_points, num_points, x, y, z,
                     time_index, gast_rad, frequency_hz, depth + 1, 0, signal,
                     status);
-            for (i = 1; i < s->num_elements; ++i)
-                oskar_mem_copy_contents(signal, signal,
-                        (size_t) i * (size_t) num_points, 0,
-                        (size_t) num_points, status);
+            child_idx = work_zero_idx(work, s->num_elements, status);
         }
         else
         {
+            signal = work_beam(work, beam,
+                    (size_t) s->num_elements * (size_t) num_points, depth,
+                    status);
             for (i = 0; i < s->num_elements; ++i)
                 aa_beam(s->child[i], work, offset_points, num_points, x, y, z,
                         time_index, gast_rad, frequency_hz, depth + 1,
@@ -1469,7 +1490,7 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
                     wavenumber, work->weights, feed_mem(s, feed, F_TRUE),
                     feed_mem(s, feed, F_TRUE + 1),
                     feed_mem(s, feed, F_TRUE + 2), offset_points, num_points,
-                    x, y, s->array_is_3d ? z : 0, 0, signal,
+                    x, y, s->array_is_3d ? z : 0, child_idx, signal,
                     feed == 0 || num_feeds == 1, feed == 1 || num_feeds == 1,
                     offset_out, beam, status);
         }
@@ -1558,8 +1579,27 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
  * beam per child for every point of the pass (the reference's
  * MAX_CHUNK_SIZE, oskar_evaluate_station_beam_aperture_array.c:21, is 49152;
  * a larger pass suits a GPU with HBM to spare and changes no result, since
- * every point is evaluated on its own). */
+ * every point is evaluated on its own).  The pass is at most
+ * CHILD_PASS_POINTS, and smaller when the most children of any station
+ * would make a depth's beams exceed CHILD_PASS_BYTES or a child offset
+ * (child index x points, an int) overflow. */
 #define CHILD_PASS_POINTS (1 << 20)
+#define CHILD_PASS_BYTES ((size_t) 1 << 30)
+
+/* Most children of any station in the tree (1 for a flat station). */
+static int max_children(const oskar_Station* s)
+{
+    int i = 0, m = 1;
+    if (!s->child) return 1;
+    m = s->num_elements > 1 ? s->num_elements : 1;
+    for (i = 0; i < s->num_elements; ++i)
+    {
+        const int c = max_children(s->child[i]);
+        if (c > m) m = c;
+        if (s->identical_children) break;
+    }
+    return m;
+}
 
 void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
         oskar_StationWork* work, int num_points, const oskar_Mem* x,
@@ -1587,10 +1627,16 @@ void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
         return;
     }
     /* Depth 0 is the element level, so the recursion starts at 1 (:52-66). */
-    for (start = 0; start < num_points; start += CHILD_PASS_POINTS)
+    const int children = max_children(s);
+    const size_t by_bytes = CHILD_PASS_BYTES /
+            ((size_t) children * oskar_mem_element_size(oskar_mem_type(beam)));
+    int pass = CHILD_PASS_POINTS;
+    if ((size_t) pass > by_bytes) pass = (int) by_bytes;
+    if (pass > INT_MAX / children) pass = INT_MAX / children;
+    if (pass < 1) pass = 1;
+    for (start = 0; start < num_points; start += pass)
     {
-        const int n = (num_points - start < CHILD_PASS_POINTS) ?
-                num_points - start : CHILD_PASS_POINTS;
+        const int n = (num_points - start < pass) ? num_points - start : pass;
         aa_beam(s, work, start, n, x, y, z, time_index, gast_rad,
                 frequency_hz, 1, start, beam, status);
     }

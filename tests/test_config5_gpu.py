@@ -2,8 +2,8 @@
 without it (the exact station beam,
 oskar_evaluate_station_beam_aperture_array.c:207-224): the 256-dipole
 station (elements uniform in a 38 m disc, beam 30 deg from zenith) over
-50-350 MHz, trained on the GPU on 2M exact-beam samples, evaluated at
-150 MHz on 1M directions above the horizon.
+50-350 MHz, trained on the GPU on 10M exact-beam samples (20,000 Adam
+steps of 8,192), evaluated at 150 MHz on 1M directions above the horizon.
 
 Reported: the beam error against the exact beam, the directions below
 1e-3 of the peak that fall back to the exact beam, and the time of the
@@ -11,7 +11,7 @@ fused inference kernel against the exact beam of this station and of
 larger stations on the same directions.
 
 Asserted (the trade-off as measured): training reduces the beam error by
-an order of magnitude against the untrained network, to below 10 % relative
+an order of magnitude against the untrained network, to below 8 % relative
 RMS; every fallback direction carries the exact beam bit for bit; the
 surrogate's time does not depend on the element count, so it is slower
 than the exact beam at 256 elements and faster than it at 16,384 elements
@@ -36,6 +36,7 @@ from workloads import elements, sky
 pytestmark = pytest.mark.gpu
 CM32 = OSKAR_SINGLE | OSKAR_COMPLEX | OSKAR_MATRIX
 FREQ = 150e6
+STEPS = 20000
 
 
 def make(ne, radius, rng):
@@ -60,7 +61,7 @@ def test_config5_surrogate_against_exact_beam(hip):
     station = make(256, 19.0, rng)
     work = StationWork(OSKAR_SINGLE, OSKAR_GPU)
     l0, m0 = 0.0, 0.5
-    tl, tm, tf, tb = beam_training_set(station, work, 40, 50000, 50e6,
+    tl, tm, tf, tb = beam_training_set(station, work, 200, 50000, 50e6,
                                        350e6, l0, m0, seed=3)
     sur = BeamSurrogate(50e6, 350e6, l0, m0, 19.0, seed=4)
     S = 1_000_000
@@ -88,7 +89,7 @@ def test_config5_surrogate_against_exact_beam(hip):
 
     err0 = beam_err()
     t0 = time.time()
-    hist = sur.train(len(tl), tl, tm, tf, tb, 8192, 6000, 2e-3, 2e-5, seed=6)
+    hist = sur.train(len(tl), tl, tm, tf, tb, 8192, STEPS, 2e-3, 2e-5, seed=6)
     sync_check("train")
     t_train = time.time() - t0
     err1 = beam_err()
@@ -107,16 +108,16 @@ def test_config5_surrogate_against_exact_beam(hip):
         ob = Mem(CM32, OSKAR_GPU, S)
         times[ne] = timed_ms(lambda: run_exact(big, wb, ob), reps=3)
     check_status(s, "exact beams")
-    print("config 5: %d training samples, 6000 steps in %.1f s (loss %.2e "
+    print("config 5: %d training samples, %d steps in %.1f s (loss %.2e "
           "-> %.2e); beam rel RMS vs exact at 150 MHz: untrained %.3e, "
           "trained %.3e; %d of 1M directions below 1e-3 of the peak take "
           "the exact beam (%d rows equal the exact beam); inference %.3f "
           "ms per 1M directions; exact beam %s ms" % (
-              len(tl), t_train, hist[:10].mean(), hist[-50:].mean(), err0,
-              err1, nfb, n_exact,
+              len(tl), STEPS, t_train, hist[:10].mean(), hist[-50:].mean(),
+              err0, err1, nfb, n_exact,
               t_sur, {k: round(v, 3) for k, v in times.items()}),
           flush=True)
-    assert err1 < 0.1 * err0 and err1 < 0.1
+    assert err1 < 0.1 * err0 and err1 < 0.08
     assert nfb > 0 and n_exact >= nfb
     assert times[16384] > 8 * times[256]
     assert t_256 < t_sur < times[16384]
