@@ -49,6 +49,41 @@ void oskar_cross_correlate(
         int* status);
 
 /*
+ * oskar_cross_correlate() without any device allocation.
+ *
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Arguments, checks, errors and results
+ * as oskar_cross_correlate(); the split-source partial sums go to a fixed
+ * 64 MiB array that every device holds until the process ends, instead of
+ * a workspace the library allocates.  No input is rejected for its size:
+ * when the partials of the usual split count do not fit, the sources are
+ * split fewer times (down to one split, which adds to vis directly and
+ * uses no scratch), so results then differ from oskar_cross_correlate()'s
+ * by the FP64 summation order only.  Up to 512 stations the split count,
+ * and so every result bit, is that of oskar_cross_correlate().  Every call
+ * on one device shares that device's array: calls that overlap on one
+ * device (two streams or two host threads) race and must be serialised by
+ * the caller.
+ */
+OSKAR_EXPORT
+void oskar_cross_correlate_no_alloc(
+        int use_casa_phase_convention,
+        int source_type,
+        int num_sources,
+        const oskar_Jones* jones,
+        const oskar_Mem* const src_flux[4],
+        const oskar_Mem* const src_dir[3],
+        const oskar_Mem* const src_ext[3],
+        const oskar_Telescope* tel,
+        const oskar_Mem* const station_uvw[3],
+        double gast,
+        double frequency_hz,
+        int offset_out,
+        oskar_Mem* vis,
+        int* status);
+
+/*
  * Backend seam: raw device pointers, launched on the current HIP stream.
  * These replace oskar_cross_correlate_{point,gaussian}_cuda_{f,d}
  * (oskar/correlate/oskar_cross_correlate_cuda.h:59-244) and

@@ -25,7 +25,7 @@ static void launch(int precision, int matrix, int gaussian,
         const void* st_v, const void* st_w, const void* st_x,
         const void* st_y, double uv_min, double uv_max, double inv_wl,
         double frac_bw, double time_int, double gha0, double dec0,
-        void* vis, int* status)
+        void* vis, int no_alloc, int* status)
 {
     oskar_XcorrParams p;
     p.use_casa = use_casa;
@@ -50,14 +50,16 @@ static void launch(int precision, int matrix, int gaussian,
         p.time_smearing = ((float) time_int != 0.0f);
     }
     {
-        const int e = oskar_amd_xcorr_launch(precision, matrix, &p, jones,
+        const int e = (no_alloc ? oskar_amd_xcorr_launch_no_alloc :
+                oskar_amd_xcorr_launch)(precision, matrix, &p, jones,
                 I, Q, U, V, l, m, n, a, b, c, st_u, st_v, st_w, st_x, st_y,
                 vis, oskar_amd_current_stream());
         if (e && !*status) *status = e;
     }
 }
 
-void oskar_cross_correlate(
+static void correlate(
+        int no_alloc,
         int use_casa_phase_convention,
         int source_type,
         int num_sources,
@@ -224,9 +226,51 @@ void oskar_cross_correlate(
                     oskar_mem_void_const(x), oskar_mem_void_const(y),
                     uv_filter_min, uv_filter_max, inv_wavelength,
                     frac_bandwidth, time_avg, gha0, dec0,
-                    oskar_mem_void(vis), status);
+                    oskar_mem_void(vis), no_alloc, status);
         }
     }
+}
+
+void oskar_cross_correlate(
+        int use_casa_phase_convention,
+        int source_type,
+        int num_sources,
+        const oskar_Jones* jones,
+        const oskar_Mem* const src_flux[4],
+        const oskar_Mem* const src_dir[3],
+        const oskar_Mem* const src_ext[3],
+        const oskar_Telescope* tel,
+        const oskar_Mem* const station_uvw[3],
+        double gast,
+        double frequency_hz,
+        int offset_out,
+        oskar_Mem* vis,
+        int* status)
+{
+    correlate(0, use_casa_phase_convention, source_type, num_sources, jones,
+            src_flux, src_dir, src_ext, tel, station_uvw, gast, frequency_hz,
+            offset_out, vis, status);
+}
+
+void oskar_cross_correlate_no_alloc(
+        int use_casa_phase_convention,
+        int source_type,
+        int num_sources,
+        const oskar_Jones* jones,
+        const oskar_Mem* const src_flux[4],
+        const oskar_Mem* const src_dir[3],
+        const oskar_Mem* const src_ext[3],
+        const oskar_Telescope* tel,
+        const oskar_Mem* const station_uvw[3],
+        double gast,
+        double frequency_hz,
+        int offset_out,
+        oskar_Mem* vis,
+        int* status)
+{
+    correlate(1, use_casa_phase_convention, source_type, num_sources, jones,
+            src_flux, src_dir, src_ext, tel, station_uvw, gast, frequency_hz,
+            offset_out, vis, status);
 }
 
 /* ---- Backend seam: raw-pointer entry points ------------------------------ */
@@ -250,7 +294,7 @@ void oskar_cross_correlate_point_hip_f(int use_casa, int num_sources,
             offset_out, d_jones, d_I, d_Q, d_U, d_V, d_l, d_m, d_n, 0, 0, 0,
             d_station_u, d_station_v, d_station_w, d_station_x, d_station_y,
             uv_min_lambda, uv_max_lambda, inv_wavelength, frac_bandwidth,
-            time_int_sec, gha0_rad, dec0_rad, d_vis, &status);
+            time_int_sec, gha0_rad, dec0_rad, d_vis, 0, &status);
     SEAM_END
 }
 
@@ -270,7 +314,7 @@ void oskar_cross_correlate_point_hip_d(int use_casa, int num_sources,
             offset_out, d_jones, d_I, d_Q, d_U, d_V, d_l, d_m, d_n, 0, 0, 0,
             d_station_u, d_station_v, d_station_w, d_station_x, d_station_y,
             uv_min_lambda, uv_max_lambda, inv_wavelength, frac_bandwidth,
-            time_int_sec, gha0_rad, dec0_rad, d_vis, &status);
+            time_int_sec, gha0_rad, dec0_rad, d_vis, 0, &status);
     SEAM_END
 }
 
@@ -290,7 +334,7 @@ void oskar_cross_correlate_gaussian_hip_f(int use_casa, int num_sources,
             offset_out, d_jones, d_I, d_Q, d_U, d_V, d_l, d_m, d_n, d_a, d_b,
             d_c, d_station_u, d_station_v, d_station_w, d_station_x,
             d_station_y, uv_min_lambda, uv_max_lambda, inv_wavelength,
-            frac_bandwidth, time_int_sec, gha0_rad, dec0_rad, d_vis, &status);
+            frac_bandwidth, time_int_sec, gha0_rad, dec0_rad, d_vis, 0, &status);
     SEAM_END
 }
 
@@ -311,7 +355,7 @@ void oskar_cross_correlate_gaussian_hip_d(int use_casa, int num_sources,
             offset_out, d_jones, d_I, d_Q, d_U, d_V, d_l, d_m, d_n, d_a, d_b,
             d_c, d_station_u, d_station_v, d_station_w, d_station_x,
             d_station_y, uv_min_lambda, uv_max_lambda, inv_wavelength,
-            frac_bandwidth, time_int_sec, gha0_rad, dec0_rad, d_vis, &status);
+            frac_bandwidth, time_int_sec, gha0_rad, dec0_rad, d_vis, 0, &status);
     SEAM_END
 }
 
@@ -329,7 +373,7 @@ void oskar_cross_correlate_scalar_point_hip_f(int use_casa, int num_sources,
             offset_out, d_jones, d_I, 0, 0, 0, d_l, d_m, d_n, 0, 0, 0,
             d_station_u, d_station_v, d_station_w, d_station_x, d_station_y,
             uv_min_lambda, uv_max_lambda, inv_wavelength, frac_bandwidth,
-            time_int_sec, gha0_rad, dec0_rad, d_vis, &status);
+            time_int_sec, gha0_rad, dec0_rad, d_vis, 0, &status);
     SEAM_END
 }
 
@@ -348,7 +392,7 @@ void oskar_cross_correlate_scalar_point_hip_d(int use_casa, int num_sources,
             offset_out, d_jones, d_I, 0, 0, 0, d_l, d_m, d_n, 0, 0, 0,
             d_station_u, d_station_v, d_station_w, d_station_x, d_station_y,
             uv_min_lambda, uv_max_lambda, inv_wavelength, frac_bandwidth,
-            time_int_sec, gha0_rad, dec0_rad, d_vis, &status);
+            time_int_sec, gha0_rad, dec0_rad, d_vis, 0, &status);
     SEAM_END
 }
 
@@ -368,7 +412,7 @@ void oskar_cross_correlate_scalar_gaussian_hip_f(int use_casa,
             offset_out, d_jones, d_I, 0, 0, 0, d_l, d_m, d_n, d_a, d_b, d_c,
             d_station_u, d_station_v, d_station_w, d_station_x, d_station_y,
             uv_min_lambda, uv_max_lambda, inv_wavelength, frac_bandwidth,
-            time_int_sec, gha0_rad, dec0_rad, d_vis, &status);
+            time_int_sec, gha0_rad, dec0_rad, d_vis, 0, &status);
     SEAM_END
 }
 
@@ -388,6 +432,6 @@ void oskar_cross_correlate_scalar_gaussian_hip_d(int use_casa,
             offset_out, d_jones, d_I, 0, 0, 0, d_l, d_m, d_n, d_a, d_b, d_c,
             d_station_u, d_station_v, d_station_w, d_station_x, d_station_y,
             uv_min_lambda, uv_max_lambda, inv_wavelength, frac_bandwidth,
-            time_int_sec, gha0_rad, dec0_rad, d_vis, &status);
+            time_int_sec, gha0_rad, dec0_rad, d_vis, 0, &status);
     SEAM_END
 }

@@ -278,6 +278,15 @@ int oskar_amd_xcorr_launch(int precision, int matrix,
         const void* a, const void* b, const void* c,
         const void* st_u, const void* st_v, const void* st_w,
         const void* st_x, const void* st_y, void* vis, void* stream);
+/* As oskar_amd_xcorr_launch, with the split partials in the device's fixed
+ * scratch array (no allocation; the split count is capped by its size). */
+int oskar_amd_xcorr_launch_no_alloc(int precision, int matrix,
+        const oskar_XcorrParams* p, const void* jones,
+        const void* I, const void* Q, const void* U, const void* V,
+        const void* l, const void* m, const void* n,
+        const void* a, const void* b, const void* c,
+        const void* st_u, const void* st_v, const void* st_w,
+        const void* st_x, const void* st_y, void* vis, void* stream);
 
 /* acorr.hip */
 int oskar_amd_acorr_launch(int precision, int matrix, int num_sources,

@@ -34,11 +34,22 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "device_common.h"
 #include "private.h"
 #include "workspace.h"
 
 using namespace oskar_amd;
+
+/* Split partials of the allocation-free entry point: a fixed 64 MiB array
+ * in every device's code object (8 splits of 512 stations' 130,816
+ * baselines x 8 FP64 values fit).  Host code never passes this name to a
+ * kernel: it is a host-side stand-in, not a device address; the device
+ * address comes from hipGetSymbolAddress (scratch_address below). */
+constexpr size_t XCORR_SCRATCH_BYTES = (size_t) 64 << 20;
+static __device__ __align__(256)
+        double scratch[XCORR_SCRATCH_BYTES / sizeof(double)];
 
 namespace {
 
@@ -412,8 +423,35 @@ void* workspace_get(size_t bytes, hipStream_t stream, hipError_t* err)
     return ws_acquire(WS_XCORR_SPLIT, dev, stream, bytes, err);
 }
 
+/* Device address of `scratch` on the current device, looked up once per
+ * device (with that device current) and kept. */
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_lock;
+void* g_scratch_ptr[MAX_DEVICES];
+
+hipError_t scratch_address(double** out)
+{
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (!g_scratch_ptr[dev])
+    {
+        void* p = nullptr;
+        err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess) return err;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (double*) g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+/* fixed: the device's scratch array (allocation-free entry point), whose
+ * capacity caps the split count; NULL: the per-stream workspace. */
 template <typename FP, bool MATRIX, bool GAUSS, bool BS, bool TS>
-hipError_t launch_t(XcorrArgs<FP> A, hipStream_t stream)
+hipError_t launch_t(XcorrArgs<FP> A, hipStream_t stream, double* fixed)
 {
     using G = Geometry<FP, MATRIX>;
     const int N = A.num_stations;
@@ -436,14 +474,22 @@ hipError_t launch_t(XcorrArgs<FP> A, hipStream_t stream)
     int splits = (2048 + valid - 1) / valid;
     const int max_splits = max(1, A.num_sources / min_per_split);
     splits = max(1, min(splits, min(max_splits, 256)));
+    const size_t nb = (size_t) N * (N - 1) / 2;
+    if (fixed)
+    {
+        /* As many splits as the scratch array holds (one split writes vis
+         * directly and needs none). */
+        const size_t cap = XCORR_SCRATCH_BYTES / (nb * G::NR * sizeof(double));
+        splits = max(1, (int) min((size_t) splits, cap));
+    }
     int per = (A.num_sources + splits - 1) / splits;
     per = ((per + G::TILE - 1) / G::TILE) * G::TILE;
     splits = (A.num_sources + per - 1) / per;
     A.src_per_split = per;
     A.num_splits = splits;
     A.partial = nullptr;
-    const size_t nb = (size_t) N * (N - 1) / 2;
-    if (splits > 1)
+    if (splits > 1 && fixed) A.partial = fixed;
+    else if (splits > 1)
     {
         hipError_t err = hipSuccess;
         A.partial = (double*) workspace_get(
@@ -463,10 +509,10 @@ hipError_t launch_t(XcorrArgs<FP> A, hipStream_t stream)
 
 template <typename FP, bool MATRIX>
 hipError_t dispatch(const XcorrArgs<FP>& A, bool gauss, bool bs, bool ts,
-        hipStream_t s)
+        hipStream_t s, double* fixed)
 {
 #define XC_CASE(G_, B_, T_) if (gauss == G_ && bs == B_ && ts == T_) \
-        return launch_t<FP, MATRIX, G_, B_, T_>(A, s);
+        return launch_t<FP, MATRIX, G_, B_, T_>(A, s, fixed);
     XC_CASE(false, false, false) XC_CASE(false, false, true)
     XC_CASE(false, true, false)  XC_CASE(false, true, true)
     XC_CASE(true, false, false)  XC_CASE(true, false, true)
@@ -481,7 +527,7 @@ hipError_t build_and_launch(int matrix, const oskar_XcorrParams* p,
         const void* V, const void* l, const void* m, const void* n,
         const void* a, const void* b, const void* c, const void* st_u,
         const void* st_v, const void* st_w, const void* st_x,
-        const void* st_y, void* vis, hipStream_t stream)
+        const void* st_y, void* vis, hipStream_t stream, bool no_alloc)
 {
     XcorrArgs<FP> A;
     A.num_sources = p->num_sources;
@@ -508,11 +554,18 @@ hipError_t build_and_launch(int matrix, const oskar_XcorrParams* p,
     A.vis = vis;
     A.partial = nullptr;
     if (A.num_stations < 2) return hipSuccess;
+    double* fixed = nullptr;
+    if (no_alloc)
+    {
+        const hipError_t err = scratch_address(&fixed);
+        if (err != hipSuccess) return err;
+    }
     if (matrix)
         return dispatch<FP, true>(A, p->gaussian != 0,
-                p->bandwidth_smearing != 0, p->time_smearing != 0, stream);
+                p->bandwidth_smearing != 0, p->time_smearing != 0, stream,
+                fixed);
     return dispatch<FP, false>(A, p->gaussian != 0,
-            p->bandwidth_smearing != 0, p->time_smearing != 0, stream);
+            p->bandwidth_smearing != 0, p->time_smearing != 0, stream, fixed);
 }
 
 } /* namespace */
@@ -528,8 +581,25 @@ extern "C" int oskar_amd_xcorr_launch(int precision, int matrix,
     if (precision == OSKAR_DOUBLE)
         return (int) build_and_launch<double>(matrix, p, jones, I, Q, U, V,
                 l, m, n, a, b, c, st_u, st_v, st_w, st_x, st_y, vis,
-                (hipStream_t) stream);
+                (hipStream_t) stream, false);
     return (int) build_and_launch<float>(matrix, p, jones, I, Q, U, V,
             l, m, n, a, b, c, st_u, st_v, st_w, st_x, st_y, vis,
-            (hipStream_t) stream);
+            (hipStream_t) stream, false);
+}
+
+extern "C" int oskar_amd_xcorr_launch_no_alloc(int precision, int matrix,
+        const oskar_XcorrParams* p, const void* jones,
+        const void* I, const void* Q, const void* U, const void* V,
+        const void* l, const void* m, const void* n,
+        const void* a, const void* b, const void* c,
+        const void* st_u, const void* st_v, const void* st_w,
+        const void* st_x, const void* st_y, void* vis, void* stream)
+{
+    if (precision == OSKAR_DOUBLE)
+        return (int) build_and_launch<double>(matrix, p, jones, I, Q, U, V,
+                l, m, n, a, b, c, st_u, st_v, st_w, st_x, st_y, vis,
+                (hipStream_t) stream, true);
+    return (int) build_and_launch<float>(matrix, p, jones, I, Q, U, V,
+            l, m, n, a, b, c, st_u, st_v, st_w, st_x, st_y, vis,
+            (hipStream_t) stream, true);
 }

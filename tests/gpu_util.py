@@ -23,8 +23,10 @@ def sync_check(where):
 
 
 def gpu_cross_correlate(prec, casa, gauss, matrix, g, bs, ts, vis_init=None,
-                        offset_out=0, uv_filter=None, num_sources=None):
-    """oskar_cross_correlate on the GPU for the inputs in dict g (golden
+                        offset_out=0, uv_filter=None, num_sources=None,
+                        entry="oskar_cross_correlate"):
+    """oskar_cross_correlate (or the entry point named by `entry`, which
+    takes the same arguments) on the GPU for the inputs in dict g (golden
     fixture layout); returns vis as a flat numpy array of reals."""
     P = prec_code(prec)
     ns = g["u"].size
@@ -55,14 +57,14 @@ def gpu_cross_correlate(prec, casa, gauss, matrix, g, bs, ts, vis_init=None,
         vis_init = np.zeros((offset_out + nb) * nr)
     vis = Mem.from_numpy(vis_init, jt)
     s = C.c_int(0)
-    lib.oskar_cross_correlate(
+    getattr(lib, entry)(
         int(casa), int(gauss), int(nsrc), jones.ptr,
         capi.MEMA(*[m.ptr for m in flux]), capi.MEM3(*[m.ptr for m in dirs]),
         capi.MEM3(*[m.ptr for m in ext]), tel.ptr,
         capi.MEM3(*[m.ptr for m in uvw]), float(g["gast"]), float(g["freq"]),
         int(offset_out), vis.ptr, C.byref(s))
-    check_status(s, "oskar_cross_correlate")
-    sync_check("oskar_cross_correlate")
+    check_status(s, entry)
+    sync_check(entry)
     return vis.numpy()
 
 
