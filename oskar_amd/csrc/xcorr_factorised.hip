@@ -1434,9 +1434,10 @@ int fact_launch_shape(int dev, int precision, int mode, int use_casa,
     return (int) hipGetLastError();
 }
 
-/* Phasors once per (station, source) into planes (1, default) or inside
- * every tile (0). */
-bool g_planes = true;
+/* Phasors once per (station, source) into planes (1) or inside every tile
+ * (0, default: at 512 stations x 1M sources the planes kernel writes 4 GB
+ * in 1.45 ms and saves only 0.25 ms of the product kernel). */
+bool g_planes = false;
 
 /* Tile shape of the one-call path: 64 x 64 with two workgroups per CU
  * measured fastest at 512 stations x 1M sources (14.3 ms against 15.7 ms
