@@ -114,13 +114,6 @@ OSKAR_EXPORT int oskar_cross_correlate_factorised_set_part_size(int sources);
  * one; a shape past the last returns -1 and changes nothing. */
 OSKAR_EXPORT int oskar_cross_correlate_factorised_set_tile_shape(int shape);
 
-/* Station phasors K_p(s) of oskar_cross_correlate_factorised(): 1 computes
- * each (station, source) phasor once per call into device planes (8 bytes
- * per station and source) that the tiles read; 0 (default) computes them
- * inside every tile that needs them (no planes; the wave-specialised shape
- * 3 always does).  Same arithmetic, same results.  Returns the previous
- * setting; a negative value only returns it. */
-OSKAR_EXPORT int oskar_cross_correlate_factorised_set_phasor_planes(int on);
 
 /* Stage timing with HIP events on the launch stream (off by default).
  * last_times() waits for the last profiled call on the current device and

@@ -30,7 +30,6 @@ void oskar_amd_fact_stations_set_profiling(int on);
 int oskar_amd_fact_stations_last_times(double* ms);
 
 void oskar_amd_fact_set_profiling(int on);
-int oskar_amd_fact_set_phasor_planes(int on);
 int oskar_amd_fact_set_tile_shape(int shape);
 int oskar_amd_fact_last_times(double* ms);
 
@@ -74,11 +73,6 @@ int oskar_cross_correlate_factorised_last_times(double ms[3])
 int oskar_cross_correlate_factorised_set_tile_shape(int shape)
 {
     return oskar_amd_fact_set_tile_shape(shape);
-}
-
-int oskar_cross_correlate_factorised_set_phasor_planes(int on)
-{
-    return oskar_amd_fact_set_phasor_planes(on);
 }
 
 int oskar_cross_correlate_factorised_set_part_size(int sources)

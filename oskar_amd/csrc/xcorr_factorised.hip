@@ -290,20 +290,11 @@ __device__ __forceinline__ float c_scale(const unsigned int* cmax)
 /* Source data one thread needs for one k-step: PS sources for its P row,
  * QS sources (with C) for its W rows.  Loaded two k-steps ahead so the
  * global-memory latency hides behind the MFMAs of the current step. */
-template <int PS, int QS, bool PRE = false>
+template <int PS, int QS>
 struct StepSrc
 {
     Src p[PS];
     Src w[QS];
-    float4 c[QS];
-};
-
-/* With precomputed phasors: the K values themselves (FP32 re, im). */
-template <int PS, int QS>
-struct StepSrc<PS, QS, true>
-{
-    float2 p[PS];
-    float2 w[QS];
     float4 c[QS];
 };
 
@@ -317,44 +308,9 @@ struct TileJob
     const Src* src;              /* per source: l, m, n - 1 (FP64) */
     const float4* cmat;          /* per source: C00, Re C01, Im C01, C11 */
     const double* st_uvw;        /* per station: U, V, W (revolutions) */
-    const float2* kpl;           /* precomputed K planes (or NULL) */
     float scale;                 /* power-of-two scale of C */
     float* out;
 };
-
-/* Precomputed phasor of (station p, source s) in the planes: one 64-byte
- * record of 8 sources per (k-step, station), stations of a k-step
- * consecutive, so a tile's k-step is a contiguous read per side. */
-__host__ __device__ inline size_t kpl_index(int N, int p, int s)
-{
-    return ((size_t) (s >> 3) * (size_t) N + (size_t) p) * 8 + (s & 7);
-}
-
-/* N consecutive float2 values of the planes at index i (N = 1, 2 or 4;
- * i a multiple of N). */
-template <int CNT>
-__device__ __forceinline__ void gld_f2(const float2* base, size_t i,
-        float2* o)
-{
-    if constexpr (CNT == 4)
-    {
-        const fv4 a = *(const GLOBAL_AS fv4*) (base + i);
-        const fv4 b = *(const GLOBAL_AS fv4*) (base + i + 2);
-        o[0] = make_float2(a.x, a.y); o[1] = make_float2(a.z, a.w);
-        o[2] = make_float2(b.x, b.y); o[3] = make_float2(b.z, b.w);
-    }
-    else if constexpr (CNT == 2)
-    {
-        const fv4 a = *(const GLOBAL_AS fv4*) (base + i);
-        o[0] = make_float2(a.x, a.y); o[1] = make_float2(a.z, a.w);
-    }
-    else
-    {
-        typedef float fv2 __attribute__((ext_vector_type(2)));
-        const fv2 a = *(const GLOBAL_AS fv2*) (base + i);
-        o[0] = make_float2(a.x, a.y);
-    }
-}
 
 template <int PS, int QS>
 __device__ __forceinline__ void load_step(const TileJob& J, int sb,
@@ -376,7 +332,7 @@ __device__ __forceinline__ void load_step(const TileJob& J, int sb,
  * threads): the station constants of its P and W rows, and the build of one
  * k-step into a stage buffer.  Stations past N get a zero mask (their rows
  * build as zeros). */
-template <typename T16, class TL, bool PRE = false>
+template <typename T16, class TL>
 struct Builder
 {
     static constexpr int TP = TL::TP, TQ = TL::TQ, TW = TL::TW;
@@ -395,15 +351,12 @@ struct Builder
         qg = bt / TQ;
         ps = min(J.p0 + pb, N - 1);
         qs = min(J.q0 + qb, N - 1);
-        if constexpr (!PRE)
-        {
-            pu = gld(J.st_uvw + 3 * ps);
-            pv = gld(J.st_uvw + 3 * ps + 1);
-            pw = gld(J.st_uvw + 3 * ps + 2);
-            qu = gld(J.st_uvw + 3 * qs);
-            qv = gld(J.st_uvw + 3 * qs + 1);
-            qw = gld(J.st_uvw + 3 * qs + 2);
-        }
+        pu = gld(J.st_uvw + 3 * ps);
+        pv = gld(J.st_uvw + 3 * ps + 1);
+        pw = gld(J.st_uvw + 3 * ps + 2);
+        qu = gld(J.st_uvw + 3 * qs);
+        qv = gld(J.st_uvw + 3 * qs + 1);
+        qw = gld(J.st_uvw + 3 * qs + 2);
         q_scale = J.scale;
         /* Position of this thread's sources in the k-step: plane half and
          * 16-bit element offset within the 16-byte entry. */
@@ -415,35 +368,21 @@ struct Builder
     }
 
     __device__ __forceinline__ void load(const TileJob& J, int sb,
-            StepSrc<PS, QS, PRE>* o) const
+            StepSrc<PS, QS>* o) const
     {
-        if constexpr (PRE)
-        {
-            /* k-steps past the last read the last one (a masked dummy). */
-            const int sbc = min(sb, s_last - (KSRC - 1));
-            gld_f2<PS>(J.kpl, kpl_index(J.N, ps, sbc) + pg * PS, o->p);
-            gld_f2<QS>(J.kpl, kpl_index(J.N, qs, sbc) + qg * QS, o->w);
-#pragma unroll
-            for (int j = 0; j < QS; ++j)
-                o->c[j] = gld_f4(J.cmat + sbc + qg * QS + j);
-        }
-        else load_step(J, sb, s_last, pg, qg, o);
+        load_step(J, sb, s_last, pg, qg, o);
     }
 
-    __device__ __forceinline__ void k_p(const StepSrc<PS, QS, PRE>& d, int j,
+    __device__ __forceinline__ void k_p(const StepSrc<PS, QS>& d, int j,
             float* re, float* im) const
     {
-        if constexpr (PRE) { *re = d.p[j].x; *im = d.p[j].y; }
-        else phasor(phase_rev(pu, pv, pw, d.p[j].l, d.p[j].m, d.p[j].n1),
-                re, im);
+        phasor(phase_rev(pu, pv, pw, d.p[j].l, d.p[j].m, d.p[j].n1), re, im);
     }
 
-    __device__ __forceinline__ void k_w(const StepSrc<PS, QS, PRE>& d, int j,
+    __device__ __forceinline__ void k_w(const StepSrc<PS, QS>& d, int j,
             float* re, float* im) const
     {
-        if constexpr (PRE) { *re = d.w[j].x; *im = d.w[j].y; }
-        else phasor(phase_rev(qu, qv, qw, d.w[j].l, d.w[j].m, d.w[j].n1),
-                re, im);
+        phasor(phase_rev(qu, qv, qw, d.w[j].l, d.w[j].m, d.w[j].n1), re, im);
     }
 
     /* Builds the P and W operand planes of the k-step starting at source sb
@@ -452,7 +391,7 @@ struct Builder
      * C = 0, so their products are exactly zero, and the rows of stations
      * past N (built from the last station's constants) are never read. */
     __device__ __forceinline__ void build(int sb,
-            const StepSrc<PS, QS, PRE>& d, uint4* st) const
+            const StepSrc<PS, QS>& d, uint4* st) const
     {
         {
             T16 xh[2 * PS], xl[2 * PS], yh[2 * PS], yl[2 * PS];
@@ -537,7 +476,7 @@ __device__ __forceinline__ void store_block(float* base, int row0, int col0,
 /* tid: the thread's index in the workgroup (a value the persistent batch
  * kernel makes opaque per work item, so that nothing derived from it is
  * hoisted out of its item loop and held in registers across it). */
-template <typename T16, class TL, bool PRE = false>
+template <typename T16, class TL>
 __device__ __forceinline__ void tile_part(const TileJob& J,
         uint4 (*lds)[TL::STAGE], int tid)
 {
@@ -545,7 +484,7 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
     constexpr int PS = TL::PS, QS = TL::QS, RF = TL::RF, CF = TL::CF;
     constexpr int P_ENTRIES = TL::P_ENTRIES;
     const int lane = tid & 63, wave = tid >> 6;
-    const Builder<T16, TL, PRE> bld(J, tid);
+    const Builder<T16, TL> bld(J, tid);
     const int s_begin = J.s_begin;
     const int nsteps = (J.s_end - s_begin + KSRC - 1) / KSRC;
 
@@ -559,7 +498,7 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
 
     const int wp = wave % TL::WR, ww = wave / TL::WR;
     const int lr = lane & 31, lh = lane >> 5;
-    StepSrc<PS, QS, PRE> cur;
+    StepSrc<PS, QS> cur;
     if (nsteps > 0)
     {
         bld.load(J, s_begin, &cur);
@@ -923,7 +862,6 @@ __device__ __forceinline__ void reduce_row(int N, int p, int tiles,
 struct FactArgs
 {
     int num_sources, num_stations, num_tiles, src_per_part, num_parts;
-    const float2* kpl;           /* phasor planes (PRE bodies) */
     const Src* src;
     const float4* cmat;
     const double* st_uvw;
@@ -934,16 +872,15 @@ struct FactArgs
 /* Tile bodies: geometry (TL), workgroup size, occupancy and LDS of the
  * one-role body (tile_part, 256 threads) and the wave-specialised body
  * (tile_part_ws, 512 threads, 64 x 64 geometry). */
-template <typename T16, class TL_, bool PRE_ = false>
+template <typename T16, class TL_>
 struct BodyOneRole
 {
     using TL = TL_;
-    static constexpr bool PRE = PRE_;
     static constexpr int THREADS_ = TL::NT, OCC = TL::OCC, STAGES = 2;
     static __device__ __forceinline__ void run(const TileJob& J,
             uint4 (*lds)[TL::STAGE], int tid)
     {
-        tile_part<T16, TL, PRE_>(J, lds, tid);
+        tile_part<T16, TL>(J, lds, tid);
     }
 };
 
@@ -951,7 +888,6 @@ template <typename T16>
 struct BodyWS
 {
     using TL = Tile1;
-    static constexpr bool PRE = false;
     static constexpr int THREADS_ = WS_THREADS, OCC = 1, STAGES = WS_STAGES;
     static __device__ __forceinline__ void run(const TileJob& J,
             uint4 (*lds)[TL::STAGE], int tid)
@@ -966,19 +902,7 @@ void fact_xcorr_kernel(FactArgs A)
 {
     using TL = typename Body::TL;
     __shared__ uint4 lds[Body::STAGES][TL::STAGE];
-    int tile = blockIdx.x, part = blockIdx.y;
-    if constexpr (Body::PRE)
-    {
-        /* One-dimensional grid mapped for the 8 XCDs: workgroup i is
-         * dispatched to XCD i % 8, and XCD x takes every tile of parts x,
-         * x + 8, ..., so the concurrent workgroups of an XCD read one
-         * part's phasor planes and sources through that XCD's L2.  Slots
-         * past the last part exit at once. */
-        const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
-        tile = local % A.num_tiles;
-        part = (local / A.num_tiles) * 8 + xcd;
-        if (part >= A.num_parts) return;
-    }
+    const int tile = blockIdx.x, part = blockIdx.y;
     int ib, jb;
     tile_coords(tile, A.num_stations, TL::TP, TL::TQ, &ib, &jb);
     TileJob J;
@@ -991,36 +915,10 @@ void fact_xcorr_kernel(FactArgs A)
     J.src = A.src;
     J.cmat = A.cmat;
     J.st_uvw = A.st_uvw;
-    J.kpl = A.kpl;
     J.scale = c_scale(A.cmax);
     J.out = A.partial + ((size_t) part * A.num_tiles + tile) *
             (size_t) (2 * TL::TP * TL::TW);
     Body::run(J, lds, threadIdx.x);
-}
-
-/* Phasor planes: K_p(s) = exp(2 pi i (U_p l + V_p m + W_p (n - 1))) of
- * every (station, padded source slot), the phase and its range reduction
- * in FP64 and the hardware sine and cosine, as the tile builders compute
- * them; record layout of kpl_index.  Consecutive threads write
- * consecutive values. */
-__global__ __launch_bounds__(256)
-void fact_phasor_kernel(int N, int S8, const double* st, const Src* src,
-        float2* kpl)
-{
-    const size_t total = (size_t) N * (size_t) S8;
-    for (size_t i = (size_t) blockIdx.x * 256 + threadIdx.x; i < total;
-            i += (size_t) gridDim.x * 256)
-    {
-        const int j = (int) (i & 7);
-        const size_t r = i >> 3;
-        const int p = (int) (r % (size_t) N);
-        const int s = (int) (r / (size_t) N) * 8 + j;
-        const Src d = gld_src(src + s);
-        float re, im;
-        phasor(phase_rev(gld(st + 3 * p), gld(st + 3 * p + 1),
-                gld(st + 3 * p + 2), d.l, d.m, d.n1), &re, &im);
-        kpl[i] = make_float2(re, im);
-    }
 }
 
 template <typename FP>
@@ -1306,15 +1204,7 @@ extern "C" int oskar_amd_fact_num_tiles(int N)
 
 namespace {
 
-/* The body of each tile shape with precomputed phasor planes (the
- * wave-specialised shape 3 computes them in its builder waves). */
-template <int SHAPE, typename T16> struct ShapeBodyPre
-{ using type = BodyOneRole<T16, typename ShapeBody<SHAPE, T16>::type::TL,
-        true>; };
-template <typename T16> struct ShapeBodyPre<3, T16>
-{ using type = BodyWS<T16>; };
-
-template <int SHAPE, bool PRE>
+template <int SHAPE>
 int fact_launch_shape(int dev, int precision, int mode, int use_casa,
         int S, int N, const void* beam, const void* I, const void* Q,
         const void* U, const void* V, const void* l, const void* m,
@@ -1322,26 +1212,20 @@ int fact_launch_shape(int dev, int precision, int mode, int use_casa,
         double frequency_hz, double uv_min, double uv_max, int offset_out,
         void* vis, int src_per_part, hipStream_t stream)
 {
-    using BodyH = typename std::conditional<PRE,
-            typename ShapeBodyPre<SHAPE, _Float16>::type,
-            typename ShapeBody<SHAPE, _Float16>::type>::type;
-    using BodyB = typename std::conditional<PRE,
-            typename ShapeBodyPre<SHAPE, __bf16>::type,
-            typename ShapeBody<SHAPE, __bf16>::type>::type;
+    using BodyH = typename ShapeBody<SHAPE, _Float16>::type;
+    using BodyB = typename ShapeBody<SHAPE, __bf16>::type;
     using TL = typename BodyH::TL;
-    constexpr bool USE_PLANES = BodyH::PRE;
     hipError_t err = hipSuccess;
     const int parts = (S + src_per_part - 1) / src_per_part;
     const int tiles = num_tiles(N, TL::TP, TL::TQ);
 
     /* Workspace 0: [cmax (256 B)][stations 3N doubles][src S x 24 B]
-     * [cmat S x 16 B][phasor planes N x S x 8 B]; workspace 1: partials. */
+     * [cmat S x 16 B]; workspace 1: partials. */
     const size_t off_st = 256;
     const size_t off_src = off_st + align256(3 * (size_t) N * 8);
     const int S8 = pad8(S);
     const size_t off_c = off_src + align256(sizeof(Src) * (size_t) S8);
-    const size_t off_k = off_c + align256(16 * (size_t) S8);
-    const size_t bytes0 = off_k + (USE_PLANES ? 8 * (size_t) N * S8 : 0);
+    const size_t bytes0 = off_c + align256(16 * (size_t) S8);
     char* w0 = (char*) ws_acquire(WS_FACT_SOURCES, dev, stream, bytes0,
             &err);
     if (err != hipSuccess) return (int) err;
@@ -1354,7 +1238,6 @@ int fact_launch_shape(int dev, int precision, int mode, int use_casa,
     double* st = (double*) (w0 + off_st);
     Src* src = (Src*) (w0 + off_src);
     float4* cmat = (float4*) (w0 + off_c);
-    float2* kpl = USE_PLANES ? (float2*) (w0 + off_k) : nullptr;
     const double f_over_c = frequency_hz / 299792458.0;
 
     mark(dev, 0, stream);
@@ -1383,14 +1266,6 @@ int fact_launch_shape(int dev, int precision, int mode, int use_casa,
                 (const float*) V, (const float*) l, (const float*) m,
                 (const float*) n, (const Cplx<float>*) beam, src, cmat, cmax);
     }
-    if (USE_PLANES)
-    {
-        const size_t total = (size_t) N * S8;
-        const unsigned int grid = (unsigned int) std::min<size_t>(
-                (total + 255) / 256, 65536);
-        hipLaunchKernelGGL(fact_phasor_kernel, dim3(grid), dim3(256), 0,
-                stream, N, S8, st, src, kpl);
-    }
     err = hipGetLastError();
     if (err != hipSuccess) return (int) err;
 
@@ -1400,17 +1275,12 @@ int fact_launch_shape(int dev, int precision, int mode, int use_casa,
     A.num_tiles = tiles;
     A.src_per_part = src_per_part;
     A.num_parts = parts;
-    A.kpl = kpl;
     A.src = src;
     A.cmat = cmat;
     A.st_uvw = st;
     A.cmax = cmax;
     A.partial = partial;
-    /* Planes: a one-dimensional grid of 8 XCD slots x tiles x part groups
-     * (see fact_xcorr_kernel); otherwise (tiles, parts). */
-    const dim3 grid = USE_PLANES ?
-            dim3((unsigned int) (8 * tiles * ((parts + 7) / 8))) :
-            dim3(tiles, parts);
+    const dim3 grid(tiles, parts);
     mark(dev, 1, stream);
     if (mode == 1)
         hipLaunchKernelGGL((fact_xcorr_kernel<BodyB>), grid,
@@ -1434,10 +1304,6 @@ int fact_launch_shape(int dev, int precision, int mode, int use_casa,
     return (int) hipGetLastError();
 }
 
-/* Phasors once per (station, source) into planes (1) or inside every tile
- * (0, default: at 512 stations x 1M sources the planes kernel writes 4 GB
- * in 1.45 ms and saves only 0.25 ms of the product kernel). */
-bool g_planes = false;
 
 /* Tile shape of the one-call path: 64 x 64 with two workgroups per CU
  * measured fastest at 512 stations x 1M sources (14.3 ms against 15.7 ms
@@ -1478,28 +1344,18 @@ extern "C" int oskar_amd_fact_launch(int precision, int mode, int use_casa,
 #define FACT_ARGS dev, precision, mode, use_casa, S, N, beam, I, Q, U, V, \
         l, m, n, st_u, st_v, st_w, frequency_hz, uv_min_lambda, \
         uv_max_lambda, offset_out, vis, src_per_part, stream
-#define FACT_SHAPES(PRE) switch (g_shape) \
-    { \
-    case 1: return fact_launch_shape<1, PRE>(FACT_ARGS); \
-    case 2: return fact_launch_shape<2, PRE>(FACT_ARGS); \
-    case 3: return fact_launch_shape<3, PRE>(FACT_ARGS); \
-    case 4: return fact_launch_shape<4, PRE>(FACT_ARGS); \
-    default: return fact_launch_shape<0, PRE>(FACT_ARGS); \
+    switch (g_shape)
+    {
+    case 1: return fact_launch_shape<1>(FACT_ARGS);
+    case 2: return fact_launch_shape<2>(FACT_ARGS);
+    case 3: return fact_launch_shape<3>(FACT_ARGS);
+    case 4: return fact_launch_shape<4>(FACT_ARGS);
+    default: return fact_launch_shape<0>(FACT_ARGS);
     }
-    if (g_planes) { FACT_SHAPES(true) }
-    FACT_SHAPES(false)
-#undef FACT_SHAPES
 #undef FACT_ARGS
 }
 
 extern "C" void oskar_amd_fact_set_profiling(int on) { g_profile = on != 0; }
-
-extern "C" int oskar_amd_fact_set_phasor_planes(int on)
-{
-    const int old = g_planes ? 1 : 0;
-    if (on >= 0) g_planes = on != 0;
-    return old;
-}
 
 /* Stage times (ms) of the last profiled call on the current device:
  * preparation, MFMA product, reduction.  Waits for the last event. */

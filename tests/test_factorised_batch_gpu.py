@@ -9,8 +9,8 @@ separate array.  Checks:
 - each unit equals the one-call path bit for bit at the same tile shape and
   part size (same arithmetic, same FP64 summation order);
 - every tile shape and part size is within the factorised tolerance of the
-  FP64 oracle of the reference's direct algorithm (1.5e-5 relative RMS, as
-  for the one-call path);
+  FP64 oracle of the reference's direct algorithm (1e-5 relative RMS, as
+  for the one-call path; 2e-5 for the 5-source unit);
 - hipGraph replays, with other GPU work between them, equal direct
   launches exactly;
 - uv filter in metres (a different wavelength limit per channel), CASA
@@ -39,7 +39,14 @@ NSRC = 16387
 UNITS = [(9001, 150.0, 0, True), (5, 50.0, 0, True), (0, 80.0, 1, True),
          (NSRC, 350.0, 1, False), (256, 100.0, 1, True),
          (4097, 220.0, 0, True)]
-TOL = 1.5e-5
+TOL = 1e-5
+# Units of fewer than 64 sources: a handful of products with nothing to
+# average over (measured 1.1e-5 on the 5-source unit at 50 MHz).
+TOL_FEW = 2e-5
+
+
+def tol(u):
+    return TOL if UNITS[u][0] >= 64 else TOL_FEW
 _SKIES = []
 _ORACLE = {}
 
@@ -186,7 +193,7 @@ def test_batch_shapes_and_parts_match_fp64_oracle(hip, shape, part, mode):
         err = rel_rms(got[u], want)
         print("  unit %d S=%d rel RMS %.3e" % (u, UNITS[u][0], err),
               flush=True)
-        assert err < TOL
+        assert err < tol(u)
 
 
 def test_batch_graph_replays_equal_direct_launch(hip):
@@ -235,7 +242,7 @@ def test_batch_uv_filter_casa_and_accumulation(hip):
                              uv_min=2000.0, uv_max=15000.0)
         untouched = want == init[u]
         assert np.array_equal(got[u][untouched], init[u][untouched])
-        assert rel_rms(got[u] - init[u], want - init[u]) < TOL
+        assert rel_rms(got[u] - init[u], want - init[u]) < tol(u)
 
 
 def test_batch_autotune_leaves_outputs_and_caches(hip):
@@ -259,7 +266,7 @@ def test_batch_autotune_leaves_outputs_and_caches(hip):
     got = case.results()
     for u in range(len(UNITS)):
         if UNITS[u][0]:
-            assert rel_rms(got[u], oracle_unit("f", u)) < TOL
+            assert rel_rms(got[u], oracle_unit("f", u)) < tol(u)
     # A second batch of the same kind reuses the choice without timing.
     other = BatchCase("f", 0)
     t0 = time.perf_counter()

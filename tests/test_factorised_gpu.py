@@ -3,11 +3,11 @@ CPU oracle of the reference's direct algorithm, through the C-ABI.
 
 The oracle forms J_p(s) = K_p(s) E(s) (oskar_evaluate_jones_K semantics)
 and runs the reference's double-precision cross-correlation on it.
-Tolerances (relative RMS of all visibility reals vs FP64 on the same
-rounded inputs):  FP16x3 and BF16x3: 1.5e-5.  The FP16 split alone would
-allow ~1e-7, but the matrix core sums the 16 products of one instruction in
-a single aligned adder, so low bits of faint sources that share a k-step
-with bright ones are truncated; measured 1e-7 .. 9e-6 on these skies.
+Tolerance (relative RMS of all visibility reals vs FP64 on the same
+rounded inputs): 1e-5 for FP16x3 and BF16x3.  Measured on an MI355X:
+FP16x3 1.5e-7 on 5 sources, 3e-6 .. 6.2e-6 on 3,000 - 20,000 sources (the
+error grows with the sources summed in FP32 per part and with the sky's
+dynamic range); BF16x3 3.9e-6 .. 8.6e-6.
 """
 import numpy as np
 import pytest
@@ -17,7 +17,7 @@ from gpu_util import gpu_factorised
 from helpers import max_rel, rel_rms
 
 pytestmark = pytest.mark.gpu
-TOL = {0: 1.5e-5, 1: 1.5e-5}
+TOL = {0: 1e-5, 1: 1e-5}
 
 
 def run(c, prec, mode, freq, casa=0, **kw):
@@ -95,25 +95,3 @@ def test_factorised_every_tile_shape(hip, shape, mode):
     finally:
         lib.oskar_cross_correlate_factorised_set_tile_shape(old)
     assert lib.oskar_cross_correlate_factorised_set_tile_shape(9) == -1
-
-
-@pytest.mark.parametrize("shape", [0, 1, 2, 4])
-def test_factorised_phasor_planes_equal_in_tile_phasors(hip, shape):
-    """Phasors computed once into planes or inside every tile (the
-    default) give the same visibilities, bit for bit (same phase arithmetic,
-    same products in the same order), for every tile shape."""
-    from oskar_amd import lib
-    c = make_case(91 + shape, 140, 7001)
-    freq = 130e6
-    old_shape = lib.oskar_cross_correlate_factorised_set_tile_shape(shape)
-    try:
-        assert lib.oskar_cross_correlate_factorised_set_phasor_planes(-1) == 0
-        lib.oskar_cross_correlate_factorised_set_phasor_planes(1)
-        a = run(c, "f", 0, freq)
-        lib.oskar_cross_correlate_factorised_set_phasor_planes(0)
-        b = run(c, "f", 0, freq)
-    finally:
-        lib.oskar_cross_correlate_factorised_set_phasor_planes(0)
-        lib.oskar_cross_correlate_factorised_set_tile_shape(old_shape)
-    assert np.array_equal(a, b)
-    assert rel_rms(a, oracle_direct(rounded(c, "f"), freq)) < TOL[0]
