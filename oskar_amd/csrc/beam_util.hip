@@ -510,6 +510,33 @@ extern "C" int oskar_amd_dipole_launch(int precision, int matrix,
     return (int) hipGetLastError();
 }
 
+/* One point (x, y, z) written at index i of three device arrays, in stream
+ * order (a host copy would wait for the device). */
+template <typename FP>
+__global__ void set_point3_kernel(FP* x, FP* y, FP* z, size_t i, double vx,
+        double vy, double vz)
+{
+    if (threadIdx.x == 0)
+    {
+        x[i] = (FP) vx;
+        y[i] = (FP) vy;
+        z[i] = (FP) vz;
+    }
+}
+
+extern "C" int oskar_amd_set_point3_launch(int precision, void* x, void* y,
+        void* z, size_t index, double vx, double vy, double vz, void* stream)
+{
+    const hipStream_t s = (hipStream_t) stream;
+    if (precision == OSKAR_DOUBLE)
+        hipLaunchKernelGGL(set_point3_kernel<double>, dim3(1), dim3(64), 0, s,
+                (double*) x, (double*) y, (double*) z, index, vx, vy, vz);
+    else
+        hipLaunchKernelGGL(set_point3_kernel<float>, dim3(1), dim3(64), 0, s,
+                (float*) x, (float*) y, (float*) z, index, vx, vy, vz);
+    return (int) hipGetLastError();
+}
+
 extern "C" int oskar_amd_blank_below_horizon_launch(int precision,
         int matrix, int offset_mask, int num, const void* mask,
         int offset_out, void* jones, void* stream)

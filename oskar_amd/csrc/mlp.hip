@@ -114,14 +114,22 @@ __device__ __forceinline__ f16v mfma_bf16(const uint4& a, const uint4& b,
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
 }
 
+/* SiLU and its derivative with the hardware reciprocal (about 1 ulp; the
+ * activations are rounded to BF16, 2^-9, right after): the IEEE division
+ * sequence would cost more than the layer's MFMAs. */
+__device__ __forceinline__ float sigmoid(float z)
+{
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+
 __device__ __forceinline__ float silu(float z)
 {
-    return z / (1.0f + __expf(-z));
+    return z * sigmoid(z);
 }
 
 __device__ __forceinline__ float dsilu(float z)
 {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = sigmoid(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -555,61 +563,72 @@ __device__ __forceinline__ void load_biases(const float* bias, int n_base,
                         n_base + 32 * i + 8 * grp + 4 * lh + q);
 }
 
-/* One layer, transposed: D[n][m] = sum_k W[n][k] a[m][k] for this wave's 64
- * rows n (2 blocks) and all 128 directions m (4 blocks); K features in LDS
- * planes act_in[k / 8][m]. */
-template <int K>
-__device__ __forceinline__ void infer_layer(const LayerRegs& r,
-        const uint4* act_in, f16v acc[2][4])
-{
-    const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
-#pragma unroll
-    for (int kk = 0; kk < K / 16; ++kk)
-    {
-        const uint4* pl = act_in + (2 * kk + lh) * INF_ROWS + lr;
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-        {
-            const uint4 b = pl[j * 32];
-            acc[0][j] = mfma_bf16(r.w[kk][0], b, acc[0][j]);
-            acc[1][j] = mfma_bf16(r.w[kk][1], b, acc[1][j]);
-        }
-    }
-}
+/* Next-layer weights that one layer's k-loop prefetches into the weight
+ * registers it has finished with: NEXT_HIDDEN for a hidden layer of WIDTH
+ * inputs (both 32-row blocks of this wave's rows), NEXT_OUT for the output
+ * layer (row lr of its 32 rows, one block), NEXT_NONE for nothing. */
+enum NextKind { NEXT_NONE = 0, NEXT_HIDDEN = 1, NEXT_OUT = 2 };
 
-/* bias + SiLU, BF16, into the next layer's planes: lane (m = 32 j + lr)
- * holds rows n = n_base + 32 i + 8 grp + 4 lh + q, i.e. 4 consecutive
- * features: one 8-byte write per (i, j, grp). */
-__device__ __forceinline__ void infer_store(const LayerRegs& r, int n_base,
-        f16v acc[2][4], uint4* act_out)
+/* One hidden layer, transposed: D[n][m] = sum_k W[n][k] a[m][k] for this
+ * wave's 64 rows n (2 blocks) and all 128 directions m (4 blocks of 32); K
+ * features in LDS planes act_in[k / 8][m].  The blocks of directions are
+ * done one after the other, each finished by its bias + SiLU epilogue
+ * (BF16, into the next layer's planes: lane (m = 32 j + lr) holds rows
+ * n = n_base + 32 i + 8 grp + 4 lh + q, i.e. 4 consecutive features, one
+ * 8-byte write per (i, grp)), so one block's epilogue can run beside the
+ * next block's MFMAs.  In the last block, after the MFMAs of k-step kk have
+ * read r.w[kk], the same registers receive k-step kk of the next layer's
+ * weights (Wn, of the kind NEXT), so the next layer finds its operands
+ * loaded. */
+template <int K, int NEXT>
+__device__ __forceinline__ void infer_layer(LayerRegs& r,
+        const uint4* act_in, uint4* act_out, const __bf16* Wn, int n_base)
 {
     const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5;
+    const __bf16* w0 = Wn + (NEXT == NEXT_OUT ? (size_t) lr * WIDTH :
+            (size_t) (n_base + lr) * WIDTH) + 8 * lh;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 4; ++j)
+    {
+        f16v acc[2];
 #pragma unroll
-        for (int grp = 0; grp < 4; ++grp)
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][v] = 0.0f;
+#pragma unroll
+        for (int kk = 0; kk < K / 16; ++kk)
         {
-            const int n = n_base + 32 * i + 8 * grp + 4 * lh;
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
+            const uint4 b = act_in[(2 * kk + lh) * INF_ROWS + 32 * j + lr];
+            acc[0] = mfma_bf16(r.w[kk][0], b, acc[0]);
+            acc[1] = mfma_bf16(r.w[kk][1], b, acc[1]);
+            if (j == 3)
             {
+                if constexpr (NEXT == NEXT_HIDDEN)
+                {
+                    r.w[kk][0] = gld16(w0 + 16 * kk);
+                    r.w[kk][1] = gld16(w0 + (size_t) 32 * WIDTH + 16 * kk);
+                }
+                else if constexpr (NEXT == NEXT_OUT)
+                    r.w[kk][0] = gld16(w0 + 16 * kk);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int grp = 0; grp < 4; ++grp)
+            {
+                const int n = n_base + 32 * i + 8 * grp + 4 * lh;
                 __bf16 h[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    h[q] = (__bf16) silu(acc[i][j][4 * grp + q] +
+                    h[q] = (__bf16) silu(acc[i][4 * grp + q] +
                             r.b[i][grp][q]);
                 uint2 pk;
                 __builtin_memcpy(&pk, h, 8);
                 reinterpret_cast<uint2*>(
                         &act_out[(n / 8) * INF_ROWS + j * 32 + lr])[lh] = pk;
             }
-        }
+    }
 }
 
 template <typename FP>
@@ -625,8 +644,20 @@ void surr_infer_kernel(int num_points, const FP* l, const FP* m,
     const int d0 = blockIdx.x * INF_ROWS;
     const int n_base = wave * 64;
     LayerRegs r;
-    /* Layer 0's operands load while the encoding is computed. */
+    /* Layer 0's operands load while the encoding is computed, and so do the
+     * k-steps of layer 1 whose registers layer 0 does not use (its K is
+     * half the width); layer 0's k-loop loads the rest of layer 1. */
     load_weights<D_IN>(wb + t.weight_off[0], n_base, r);
+    {
+        const __bf16* w1 = wb + t.weight_off[1] +
+                (size_t) (n_base + lr) * WIDTH + 8 * lh;
+#pragma unroll
+        for (int kk = D_IN / 16; kk < WIDTH / 16; ++kk)
+        {
+            r.w[kk][0] = gld16(w1 + 16 * kk);
+            r.w[kk][1] = gld16(w1 + (size_t) 32 * WIDTH + 16 * kk);
+        }
+    }
     load_biases(params + t.param_off[0] + WIDTH * D_IN, n_base, r);
 
     /* Encoding: two threads per direction, 64 features each (the half is
@@ -652,31 +683,30 @@ void surr_infer_kernel(int num_points, const FP* l, const FP* m,
         }
     }
     __syncthreads();
-    f16v acc[2][4];
 #pragma unroll
     for (int layer = 0; layer < DEPTH; ++layer)
     {
         const uint4* in = act[layer & 1];
         uint4* out = act[(layer + 1) & 1];
-        if (layer == 0) infer_layer<D_IN>(r, in, acc);
-        else infer_layer<WIDTH>(r, in, acc);
-        /* The next layer's weights load during this layer's epilogue and
-         * barrier (the MFMAs above have consumed the registers). */
-        if (layer + 1 < DEPTH)
-            load_weights<WIDTH>(wb + t.weight_off[layer + 1], n_base, r);
-        infer_store(r, n_base, acc, out);
+        /* Each layer's k-loop loads the next layer's weights (the output
+         * layer's after the last hidden layer). */
+        const __bf16* wn = wb + t.weight_off[layer + 1];
+        if (layer == 0)
+            infer_layer<D_IN, NEXT_HIDDEN>(r, in, out, wn, n_base);
+        else if (layer + 1 < DEPTH)
+            infer_layer<WIDTH, NEXT_HIDDEN>(r, in, out, wn, n_base);
+        else
+            infer_layer<WIDTH, NEXT_OUT>(r, in, out, wn, n_base);
+        /* The biases of the next layer (the registers are free now). */
         if (layer + 1 < DEPTH)
             load_biases(params + t.param_off[layer + 1] + WIDTH * WIDTH,
                     n_base, r);
         __syncthreads();
     }
-    /* Output layer: 32 rows (8 used) for this wave's 32 directions. */
+    /* Output layer: 32 rows (8 used) for this wave's 32 directions; its
+     * weights were loaded by the last hidden layer's k-loop. */
     {
         const uint4* in = act[DEPTH & 1];
-        const __bf16* w = wb + t.weight_off[DEPTH] + (size_t) lr * WIDTH +
-                8 * lh;
-#pragma unroll
-        for (int kk = 0; kk < WIDTH / 16; ++kk) r.w[kk][0] = gld16(w + 16 * kk);
         f16v o;
 #pragma unroll
         for (int v = 0; v < 16; ++v) o[v] = 0.0f;

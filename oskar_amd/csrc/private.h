@@ -331,6 +331,8 @@ int oskar_amd_dftw_launch(int precision, int matrix, int is_3d,
         int offset_out, void* output, double norm_factor, void* stream);
 
 /* beam_util.hip */
+int oskar_amd_set_point3_launch(int precision, void* x, void* y, void* z,
+        size_t index, double vx, double vy, double vz, void* stream);
 int oskar_amd_element_weights_dft_launch(int precision, int num_elements,
         const void* x, const void* y, const void* z, const void* cable,
         double wavenumber, double x1, double y1, double z1, void* weights,

@@ -223,9 +223,13 @@ void oskar_station_beam(oskar_Station* station, oskar_StationWork* work,
                 lst_rad, lat_rad, &nx, &ny, &nz, status);
         for (i = 0; i < 3; ++i)
             oskar_mem_ensure(work->enu[i], (size_t) n + 1, status);
-        oskar_mem_set_element_real(work->enu[0], (size_t) n, nx, status);
-        oskar_mem_set_element_real(work->enu[1], (size_t) n, ny, status);
-        oskar_mem_set_element_real(work->enu[2], (size_t) n, nz, status);
+        if (!*status)
+            *status = oskar_amd_hip_status(oskar_amd_set_point3_launch(
+                    oskar_mem_precision(work->enu[0]),
+                    oskar_mem_void(work->enu[0]),
+                    oskar_mem_void(work->enu[1]),
+                    oskar_mem_void(work->enu[2]), (size_t) n, nx, ny, nz,
+                    oskar_amd_current_stream()));
         n++;
     }
     if (*status) return;
