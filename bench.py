@@ -151,6 +151,9 @@ def main():
     ap.add_argument("--station-steps", type=int, default=1,
                     help="distinct-layout leg: 512 station beams + the "
                          "per-station factorised product (0 = skip)")
+    ap.add_argument("--station-part", type=int, default=0,
+                    help="sources per part of the per-station factorised "
+                         "product (0 = the library's default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--batch-units", type=int, default=16,
@@ -1369,6 +1372,9 @@ def bench_stations(args, st, sky, dirs, flux, uvw, mode, timed, evals):
         step_E()
         step_fs()
 
+    if args.station_part > 0:
+        lib.oskar_cross_correlate_factorised_jones_set_part_size(
+            args.station_part)
     t_E = timed(step_E, 2, 1, "jones E (512 distinct beams)") / 2
     t_fs = timed(step_fs, args.steps, 1, "factorised jones") / args.steps
     lib.oskar_cross_correlate_factorised_set_profiling(1)

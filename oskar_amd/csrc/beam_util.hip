@@ -274,6 +274,61 @@ __global__ void normalise_kernel(unsigned int offset, unsigned int n,
     }
 }
 
+/* normalise_kernel's scaling of a[0 .. n) by element idx, written to
+ * out[0 .. n) instead of in place (same products). */
+template <typename FP, int NR>
+__global__ void normalise_copy_kernel(unsigned int n, const FP* a,
+        unsigned int idx, FP* out)
+{
+    const FP* v = a + (size_t) idx * NR;
+    FP scal;
+    if (NR == 1) scal = (FP) 1 / v[0];
+    else
+    {
+        FP amp = (FP) 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) amp += v[r] * v[r];
+        if (NR == 8) amp /= (FP) 2;
+        scal = (FP) 1 / sqrt(amp);
+    }
+    GRID_STRIDE(i, (size_t) n)
+    {
+        const FP* x = a + i * NR;
+        FP* o = out + i * NR;
+        if (i == idx)
+        {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) o[r] = x[r];
+        }
+        else
+        {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) o[r] = x[r] * scal;
+        }
+    }
+}
+
+/* Three direction arrays of n values copied, and the point (px, py, pz)
+ * written after them at index n when `extra` is set. */
+template <typename FP>
+__global__ void copy3_point_kernel(unsigned int n, const FP* x, const FP* y,
+        const FP* z, FP* xo, FP* yo, FP* zo, int extra, double px, double py,
+        double pz)
+{
+    GRID_STRIDE(i, (size_t) n)
+    {
+        xo[i] = x[i];
+        yo[i] = y[i];
+        zo[i] = z[i];
+    }
+    if (extra && blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        xo[n] = (FP) px;
+        yo[n] = (FP) py;
+        zo[n] = (FP) pz;
+    }
+}
+
 template <typename FP, int NR>
 __global__ void set_value_kernel(size_t offset, size_t n, FP value, FP* a)
 {
@@ -596,6 +651,47 @@ extern "C" int oskar_amd_mem_normalise_launch(int type, unsigned int offset,
     default:
         return OSKAR_ERR_BAD_DATA_TYPE;
     }
+    return (int) hipGetLastError();
+}
+
+extern "C" int oskar_amd_normalise_copy_launch(int type, unsigned int n,
+        const void* data, unsigned int idx, void* out, void* stream)
+{
+    if (n == 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+#define NC_CASE(CODE, FP, NR) case CODE: \
+        hipLaunchKernelGGL((normalise_copy_kernel<FP, NR>), grid_for(n), \
+                dim3(THREADS), 0, s, n, (const FP*) data, idx, (FP*) out); \
+        break;
+    switch (type)
+    {
+    NC_CASE(OSKAR_SINGLE, float, 1)
+    NC_CASE(OSKAR_DOUBLE, double, 1)
+    NC_CASE(OSKAR_SINGLE_COMPLEX, float, 2)
+    NC_CASE(OSKAR_DOUBLE_COMPLEX, double, 2)
+    NC_CASE(OSKAR_SINGLE_COMPLEX_MATRIX, float, 8)
+    NC_CASE(OSKAR_DOUBLE_COMPLEX_MATRIX, double, 8)
+    default:
+        return OSKAR_ERR_BAD_DATA_TYPE;
+    }
+#undef NC_CASE
+    return (int) hipGetLastError();
+}
+
+extern "C" int oskar_amd_copy3_point_launch(int precision, unsigned int n,
+        const void* x, const void* y, const void* z, void* xo, void* yo,
+        void* zo, int extra, double px, double py, double pz, void* stream)
+{
+    const hipStream_t s = (hipStream_t) stream;
+    const dim3 g = grid_for(n > 0 ? n : 1);
+    if (precision == OSKAR_DOUBLE)
+        hipLaunchKernelGGL(copy3_point_kernel<double>, g, dim3(THREADS), 0, s,
+                n, (const double*) x, (const double*) y, (const double*) z,
+                (double*) xo, (double*) yo, (double*) zo, extra, px, py, pz);
+    else
+        hipLaunchKernelGGL(copy3_point_kernel<float>, g, dim3(THREADS), 0, s,
+                n, (const float*) x, (const float*) y, (const float*) z,
+                (float*) xo, (float*) yo, (float*) zo, extra, px, py, pz);
     return (int) hipGetLastError();
 }
 

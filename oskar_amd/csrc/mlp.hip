@@ -569,10 +569,10 @@ __device__ __forceinline__ void load_biases(const float* bias, int n_base,
  * layer (row lr of its 32 rows, one block), NEXT_NONE for nothing. */
 enum NextKind { NEXT_NONE = 0, NEXT_HIDDEN = 1, NEXT_OUT = 2 };
 
-/* One hidden layer, transposed: D[n][m] = sum_k W[n][k] a[m][k] for this
- * wave's 64 rows n (2 blocks) and all 128 directions m (4 blocks of 32); K
- * features in LDS planes act_in[k / 8][m].  The blocks of directions are
- * done one after the other, each finished by its bias + SiLU epilogue
+/* One hidden layer, transposed: D[n][m] = b[n] + sum_k W[n][k] a[m][k] for
+ * this wave's 64 rows n (2 blocks) and all 128 directions m (4 blocks of
+ * 32); K features in LDS planes act_in[k / 8][m].  The blocks of directions
+ * are done one after the other, each finished by its SiLU epilogue
  * (BF16, into the next layer's planes: lane (m = 32 j + lr) holds rows
  * n = n_base + 32 i + 8 grp + 4 lh + q, i.e. 4 consecutive features, one
  * 8-byte write per (i, grp)), so one block's epilogue can run beside the
@@ -582,7 +582,8 @@ enum NextKind { NEXT_NONE = 0, NEXT_HIDDEN = 1, NEXT_OUT = 2 };
  * loaded. */
 template <int K, int NEXT>
 __device__ __forceinline__ void infer_layer(LayerRegs& r,
-        const uint4* act_in, uint4* act_out, const __bf16* Wn, int n_base)
+        const uint4* act_in, uint4* act_out, const __bf16* Wn,
+        const float* bn, int n_base)
 {
     const int lane = threadIdx.x & 63, lr = lane & 31, lh = lane >> 5;
     const __bf16* w0 = Wn + (NEXT == NEXT_OUT ? (size_t) lr * WIDTH :
@@ -590,11 +591,16 @@ __device__ __forceinline__ void infer_layer(LayerRegs& r,
 #pragma unroll
     for (int j = 0; j < 4; ++j)
     {
+        /* The accumulators start at the rows' biases. */
         f16v acc[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int v = 0; v < 16; ++v) acc[i][v] = 0.0f;
+            for (int v = 0; v < 16; ++v) acc[i][v] = r.b[i][v / 4][v % 4];
+        /* The last block's accumulators hold the biases now: load the next
+         * hidden layer's into the registers. */
+        if constexpr (NEXT == NEXT_HIDDEN)
+            if (j == 3) load_biases(bn, n_base, r);
 #pragma unroll
         for (int kk = 0; kk < K / 16; ++kk)
         {
@@ -621,8 +627,7 @@ __device__ __forceinline__ void infer_layer(LayerRegs& r,
                 __bf16 h[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    h[q] = (__bf16) silu(acc[i][4 * grp + q] +
-                            r.b[i][grp][q]);
+                    h[q] = (__bf16) silu(acc[i][4 * grp + q]);
                 uint2 pk;
                 __builtin_memcpy(&pk, h, 8);
                 reinterpret_cast<uint2*>(
@@ -691,16 +696,13 @@ void surr_infer_kernel(int num_points, const FP* l, const FP* m,
         /* Each layer's k-loop loads the next layer's weights (the output
          * layer's after the last hidden layer). */
         const __bf16* wn = wb + t.weight_off[layer + 1];
+        const float* bn = params + t.param_off[layer + 1] + WIDTH * WIDTH;
         if (layer == 0)
-            infer_layer<D_IN, NEXT_HIDDEN>(r, in, out, wn, n_base);
+            infer_layer<D_IN, NEXT_HIDDEN>(r, in, out, wn, bn, n_base);
         else if (layer + 1 < DEPTH)
-            infer_layer<WIDTH, NEXT_HIDDEN>(r, in, out, wn, n_base);
+            infer_layer<WIDTH, NEXT_HIDDEN>(r, in, out, wn, bn, n_base);
         else
-            infer_layer<WIDTH, NEXT_OUT>(r, in, out, wn, n_base);
-        /* The biases of the next layer (the registers are free now). */
-        if (layer + 1 < DEPTH)
-            load_biases(params + t.param_off[layer + 1] + WIDTH * WIDTH,
-                    n_base, r);
+            infer_layer<WIDTH, NEXT_OUT>(r, in, out, wn, bn, n_base);
         __syncthreads();
     }
     /* Output layer: 32 rows (8 used) for this wave's 32 directions; its

@@ -333,6 +333,11 @@ int oskar_amd_dftw_launch(int precision, int matrix, int is_3d,
 /* beam_util.hip */
 int oskar_amd_set_point3_launch(int precision, void* x, void* y, void* z,
         size_t index, double vx, double vy, double vz, void* stream);
+int oskar_amd_normalise_copy_launch(int type, unsigned int n,
+        const void* data, unsigned int idx, void* out, void* stream);
+int oskar_amd_copy3_point_launch(int precision, unsigned int n,
+        const void* x, const void* y, const void* z, void* xo, void* yo,
+        void* zo, int extra, double px, double py, double pz, void* stream);
 int oskar_amd_element_weights_dft_launch(int precision, int num_elements,
         const void* x, const void* y, const void* z, const void* cable,
         double wavenumber, double x1, double y1, double z1, void* weights,

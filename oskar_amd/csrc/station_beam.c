@@ -209,29 +209,55 @@ void oskar_station_beam(oskar_Station* station, oskar_StationWork* work,
     }
     if (num_points == 0) return;
 
-    /* Source directions, then the normalisation point at index n. */
-    oskar_convert_any_to_enu_directions(source_coord_type, num_points,
-            source_coords, ref_lon_rad, ref_lat_rad, lst_rad, lat_rad,
-            work->enu, status);
-    if (*status) return;
+    /* Source directions, then the normalisation point at index n (for
+     * directions given as ENU already, one kernel copies them and writes
+     * the point). */
     const int normalise = oskar_station_normalise_final_beam(station) &&
             type != OSKAR_STATION_TYPE_ISOTROPIC;
+    double nx = 0, ny = 0, nz = 0;
     if (normalise)
-    {
-        double nx = 0, ny = 0, nz = 0;
         one_direction_to_enu(norm_coord_type, norm_lon_rad, norm_lat_rad,
                 lst_rad, lat_rad, &nx, &ny, &nz, status);
+    int enu_on_device = (source_coord_type == OSKAR_COORDS_ENU_DIR);
+    for (i = 0; i < 3 && enu_on_device; ++i)
+        enu_on_device = oskar_mem_location(source_coords[i]) == OSKAR_GPU &&
+                oskar_mem_type(source_coords[i]) ==
+                oskar_mem_type(work->enu[i]) &&
+                oskar_mem_length(source_coords[i]) >= (size_t) n;
+    if (enu_on_device)
+    {
         for (i = 0; i < 3; ++i)
             oskar_mem_ensure(work->enu[i], (size_t) n + 1, status);
         if (!*status)
-            *status = oskar_amd_hip_status(oskar_amd_set_point3_launch(
-                    oskar_mem_precision(work->enu[0]),
+            *status = oskar_amd_hip_status(oskar_amd_copy3_point_launch(
+                    oskar_mem_precision(work->enu[0]), (unsigned int) n,
+                    oskar_mem_void_const(source_coords[0]),
+                    oskar_mem_void_const(source_coords[1]),
+                    oskar_mem_void_const(source_coords[2]),
                     oskar_mem_void(work->enu[0]),
                     oskar_mem_void(work->enu[1]),
-                    oskar_mem_void(work->enu[2]), (size_t) n, nx, ny, nz,
+                    oskar_mem_void(work->enu[2]), normalise, nx, ny, nz,
                     oskar_amd_current_stream()));
-        n++;
     }
+    else
+    {
+        oskar_convert_any_to_enu_directions(source_coord_type, num_points,
+                source_coords, ref_lon_rad, ref_lat_rad, lst_rad, lat_rad,
+                work->enu, status);
+        if (normalise)
+        {
+            for (i = 0; i < 3; ++i)
+                oskar_mem_ensure(work->enu[i], (size_t) n + 1, status);
+            if (!*status)
+                *status = oskar_amd_hip_status(oskar_amd_set_point3_launch(
+                        oskar_mem_precision(work->enu[0]),
+                        oskar_mem_void(work->enu[0]),
+                        oskar_mem_void(work->enu[1]),
+                        oskar_mem_void(work->enu[2]), (size_t) n, nx, ny,
+                        nz, oskar_amd_current_stream()));
+        }
+    }
+    if (normalise) n++;
     if (*status) return;
     if (!work->beam_out ||
             oskar_mem_type(work->beam_out) != oskar_mem_type(beam))
@@ -248,9 +274,18 @@ void oskar_station_beam(oskar_Station* station, oskar_StationWork* work,
         oskar_evaluate_station_beam_aperture_array(station, work, n,
                 work->enu[0], work->enu[1], work->enu[2], time_index,
                 gast_rad, frequency_hz, work->beam_out, status);
-    if (normalise)
-        oskar_mem_normalise(work->beam_out, 0, (size_t) n, (size_t) n - 1,
-                status);
-    oskar_mem_copy_contents(beam, work->beam_out, (size_t) offset_out, 0,
-            (size_t) num_points, status);
+    if (normalise && !*status)
+    {
+        /* Divided by the beam at the normalisation point (the last one)
+         * while copied to the output (oskar_mem_normalise's arithmetic). */
+        const size_t es = oskar_mem_element_size(oskar_mem_type(beam));
+        *status = oskar_amd_hip_status(oskar_amd_normalise_copy_launch(
+                oskar_mem_type(beam), (unsigned int) num_points,
+                oskar_mem_void_const(work->beam_out), (unsigned int) (n - 1),
+                (char*) oskar_mem_void(beam) + (size_t) offset_out * es,
+                oskar_amd_current_stream()));
+    }
+    else
+        oskar_mem_copy_contents(beam, work->beam_out, (size_t) offset_out, 0,
+                (size_t) num_points, status);
 }

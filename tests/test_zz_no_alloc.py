@@ -13,6 +13,11 @@ partials, which must not leak into the second's result):
 Both also accumulate into non-zero visibilities at an offset, and the
 check order and error codes of oskar_cross_correlate hold (an FP32 vis for
 FP64 Jones is a type mismatch, with vis untouched).
+
+At 600 stations the usual 6 splits of FP64 matrix partials (69 MB) do not
+fit the array, so the entry point splits the sources 5 times: the result
+equals oskar_cross_correlate's to the FP64 summation order (max difference
+below 1e-12 of the largest value).
 """
 import ctypes as C
 import math
@@ -96,3 +101,13 @@ def test_no_alloc_checks_like_oskar_cross_correlate(hip):
         C.byref(s))
     assert s.value == -9            # OSKAR_ERR_TYPE_MISMATCH
     assert np.array_equal(vis.numpy(), init.astype(np.float32))
+
+
+def test_no_alloc_fewer_splits_when_scratch_is_short(hip):
+    rng = np.random.default_rng(600)
+    print("no_alloc capped: 600 st x 2000 src FP64 matrix Gaussian BS TS",
+          flush=True)
+    got, want, init = both("d", 1, 1, inputs(rng, 600, 2000), 1, 1, 0)
+    scale = float(np.max(np.abs(want - init)))
+    assert scale > 0.0
+    assert float(np.max(np.abs(got - want))) <= 1e-12 * scale
