@@ -149,23 +149,32 @@ __global__ void weights_scale_kernel(int n, Cplx<FP>* w, ErrArgs<FP> e)
     }
 }
 
+/* theta and the two phi (each offset by its delta, in [0, 2 pi)) of one
+ * ENU direction. */
+template <typename FP>
+__device__ __forceinline__ void theta_phi_point(FP xx, FP yy, FP zz,
+        FP delta_phi1, FP delta_phi2, FP* theta, FP* phi1, FP* phi2)
+{
+    const FP twopi = 2 * pi_v<FP>();
+    const FP p0 = atan2(yy, xx);
+    FP p1 = fmod(p0 + delta_phi1, twopi);
+    FP p2 = fmod(p0 + delta_phi2, twopi);
+    if (p1 < (FP) 0) p1 += twopi;
+    if (p2 < (FP) 0) p2 += twopi;
+    *theta = atan2(sqrt(xx * xx + yy * yy), zz);
+    *phi1 = p1;
+    *phi2 = p2;
+}
+
 template <typename FP>
 __global__ void theta_phi_kernel(int off_in, int num, const FP* x,
         const FP* y, const FP* z, int extra_point_at_pole, FP delta_phi1,
         FP delta_phi2, FP* theta, FP* phi1, FP* phi2)
 {
-    const FP twopi = 2 * pi_v<FP>();
     GRID_STRIDE(i, (size_t) num)
     {
-        const FP xx = x[i + off_in], yy = y[i + off_in], zz = z[i + off_in];
-        const FP p0 = atan2(yy, xx);
-        FP p1 = fmod(p0 + delta_phi1, twopi);
-        FP p2 = fmod(p0 + delta_phi2, twopi);
-        if (p1 < (FP) 0) p1 += twopi;
-        if (p2 < (FP) 0) p2 += twopi;
-        theta[i] = atan2(sqrt(xx * xx + yy * yy), zz);
-        phi1[i] = p1;
-        phi2[i] = p2;
+        theta_phi_point<FP>(x[i + off_in], y[i + off_in], z[i + off_in],
+                delta_phi1, delta_phi2, &theta[i], &phi1[i], &phi2[i]);
     }
     if (extra_point_at_pole && blockIdx.x == 0 && threadIdx.x == 0)
     {
@@ -197,6 +206,39 @@ __device__ __forceinline__ void dipole(FP phi, FP sin_theta, FP cos_theta,
     }
 }
 
+/* The dipole pattern of one direction, written to pattern[o]: the 2x2
+ * matrix (X theta, X phi; Y theta, Y phi), rows swapped with swap_xy, or
+ * for a scalar pattern the amplitude sqrt(sum of squares / 2). */
+template <typename FP, bool MATRIX>
+__device__ __forceinline__ void dipole_point(FP theta, FP phi_x, FP phi_y,
+        FP kL, FP cos_kL, int swap_xy, Cplx<FP>* pattern, size_t o)
+{
+    FP sin_theta, cos_theta;
+    sincos_v<FP>(theta, &sin_theta, &cos_theta);
+    Cplx<FP> x_th, x_ph, y_th, y_ph;
+    dipole<FP>(phi_x, sin_theta, cos_theta, kL, cos_kL, &x_th, &x_ph);
+    dipole<FP>(phi_y, sin_theta, cos_theta, kL, cos_kL, &y_th, &y_ph);
+    if (MATRIX)
+    {
+        Cplx<FP>* p = pattern + 4 * o;
+        if (swap_xy)
+        {
+            p[0] = y_th; p[1] = y_ph; p[2] = x_th; p[3] = x_ph;
+        }
+        else
+        {
+            p[0] = x_th; p[1] = x_ph; p[2] = y_th; p[3] = y_ph;
+        }
+    }
+    else
+    {
+        const FP amp = sqrt((x_th.x * x_th.x + x_ph.x * x_ph.x +
+                y_th.x * y_th.x + y_ph.x * y_ph.x) / (FP) 2);
+        pattern[o].x = amp;
+        pattern[o].y = (FP) 0;
+    }
+}
+
 template <typename FP, bool MATRIX>
 __global__ void dipole_kernel(int n, const FP* theta, const FP* phi_x,
         const FP* phi_y, FP kL, FP cos_kL, int swap_xy, int offset_out,
@@ -204,30 +246,29 @@ __global__ void dipole_kernel(int n, const FP* theta, const FP* phi_x,
 {
     GRID_STRIDE(i, (size_t) n)
     {
-        FP sin_theta, cos_theta;
-        sincos_v<FP>(theta[i], &sin_theta, &cos_theta);
-        Cplx<FP> x_th, x_ph, y_th, y_ph;
-        dipole<FP>(phi_x[i], sin_theta, cos_theta, kL, cos_kL, &x_th, &x_ph);
-        dipole<FP>(phi_y[i], sin_theta, cos_theta, kL, cos_kL, &y_th, &y_ph);
-        if (MATRIX)
-        {
-            Cplx<FP>* o = pattern + 4 * (i + offset_out);
-            if (swap_xy)
-            {
-                o[0] = y_th; o[1] = y_ph; o[2] = x_th; o[3] = x_ph;
-            }
-            else
-            {
-                o[0] = x_th; o[1] = x_ph; o[2] = y_th; o[3] = y_ph;
-            }
-        }
-        else
-        {
-            const FP amp = sqrt((x_th.x * x_th.x + x_ph.x * x_ph.x +
-                    y_th.x * y_th.x + y_ph.x * y_ph.x) / (FP) 2);
-            pattern[i + offset_out].x = amp;
-            pattern[i + offset_out].y = (FP) 0;
-        }
+        dipole_point<FP, MATRIX>(theta[i], phi_x[i], phi_y[i], kL, cos_kL,
+                swap_xy, pattern, i + offset_out);
+    }
+}
+
+/* theta_phi_kernel and dipole_kernel in one pass, theta and phi kept in
+ * registers: the pattern of directions [off_in, off_in + num), then, with
+ * extra_point_at_pole, the pole (theta 0, phi = the deltas) at index num. */
+template <typename FP, bool MATRIX>
+__global__ void dipole_enu_kernel(int off_in, int num, const FP* x,
+        const FP* y, const FP* z, int extra_point_at_pole, FP delta_phi1,
+        FP delta_phi2, FP kL, FP cos_kL, int swap_xy, int offset_out,
+        Cplx<FP>* pattern)
+{
+    const size_t total = (size_t) num + (extra_point_at_pole ? 1 : 0);
+    GRID_STRIDE(i, total)
+    {
+        FP theta = (FP) 0, p1 = delta_phi1, p2 = delta_phi2;
+        if (i < (size_t) num)
+            theta_phi_point<FP>(x[i + off_in], y[i + off_in], z[i + off_in],
+                    delta_phi1, delta_phi2, &theta, &p1, &p2);
+        dipole_point<FP, MATRIX>(theta, p1, p2, kL, cos_kL, swap_xy,
+                pattern, i + offset_out);
     }
 }
 
@@ -274,38 +315,81 @@ __global__ void normalise_kernel(unsigned int offset, unsigned int n,
     }
 }
 
-/* normalise_kernel's scaling of a[0 .. n) by element idx, written to
- * out[0 .. n) instead of in place (same products). */
-template <typename FP, int NR>
-__global__ void normalise_copy_kernel(unsigned int n, const FP* a,
-        unsigned int idx, FP* out)
+/* The last steps of a station beam, in one pass: the horizon blank of
+ * blank_kernel (elements whose mask is negative become zeros) and
+ * normalise_kernel's scaling by element idx (blanked alike), written to
+ * out[0 .. n) (n <= idx).  Without NORM the blanked values are copied.
+ * Elements move as NV vectors of type V (16 bytes when the pointers allow
+ * it); the products are normalise_kernel's. */
+template <typename FP, int NR, typename V, bool NORM, bool BLANK>
+__global__ void beam_finish_kernel(unsigned int n, const FP* a,
+        unsigned int idx, const FP* mask, FP* out)
 {
-    const FP* v = a + (size_t) idx * NR;
-    FP scal;
-    if (NR == 1) scal = (FP) 1 / v[0];
-    else
+    constexpr int NV = NR * (int) sizeof(FP) / (int) sizeof(V);
+    static_assert(NV * sizeof(V) == NR * sizeof(FP), "vector size");
+    FP scal = (FP) 1;
+    if constexpr (NORM)
     {
-        FP amp = (FP) 0;
+        FP v[NR];
+        const bool zero = BLANK && mask[idx] < (FP) 0;
 #pragma unroll
-        for (int r = 0; r < NR; ++r) amp += v[r] * v[r];
-        if (NR == 8) amp /= (FP) 2;
-        scal = (FP) 1 / sqrt(amp);
+        for (int r = 0; r < NR; ++r)
+            v[r] = zero ? (FP) 0 : a[(size_t) idx * NR + r];
+        if (NR == 1) scal = (FP) 1 / v[0];
+        else
+        {
+            FP amp = (FP) 0;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) amp += v[r] * v[r];
+            if (NR == 8) amp /= (FP) 2;
+            scal = (FP) 1 / sqrt(amp);
+        }
     }
     GRID_STRIDE(i, (size_t) n)
     {
-        const FP* x = a + i * NR;
-        FP* o = out + i * NR;
-        if (i == idx)
-        {
+        V buf[NV];
+        const V* src = (const V*) a + i * NV;
 #pragma unroll
-            for (int r = 0; r < NR; ++r) o[r] = x[r];
-        }
-        else
-        {
+        for (int k = 0; k < NV; ++k) buf[k] = src[k];
+        FP x[NR];
+        __builtin_memcpy(x, buf, sizeof(x));
+        const bool zero = BLANK && mask[i] < (FP) 0;
 #pragma unroll
-            for (int r = 0; r < NR; ++r) o[r] = x[r] * scal;
+        for (int r = 0; r < NR; ++r)
+        {
+            FP y = zero ? (FP) 0 : x[r];
+            if constexpr (NORM) y = y * scal;
+            x[r] = y;
         }
+        __builtin_memcpy(buf, x, sizeof(x));
+        V* dst = (V*) out + i * NV;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) dst[k] = buf[k];
     }
+}
+
+template <typename FP, int NR, typename V>
+int beam_finish_t(unsigned int n, const void* data, unsigned int idx,
+        int normalise, const void* mask, void* out, hipStream_t s)
+{
+    const FP* a = (const FP*) data;
+    const FP* m = (const FP*) mask;
+    FP* o = (FP*) out;
+#define BF(NORM, BLANK) hipLaunchKernelGGL( \
+        (beam_finish_kernel<FP, NR, V, NORM, BLANK>), grid_for(n), \
+        dim3(THREADS), 0, s, n, a, idx, m, o)
+    if (normalise)
+    {
+        if (mask) BF(true, true);
+        else BF(true, false);
+    }
+    else
+    {
+        if (mask) BF(false, true);
+        else BF(false, false);
+    }
+#undef BF
+    return (int) hipGetLastError();
 }
 
 /* Three direction arrays of n values copied, and the point (px, py, pz)
@@ -525,6 +609,34 @@ extern "C" int oskar_amd_theta_phi_launch(int precision, int offset_in,
     return (int) hipGetLastError();
 }
 
+extern "C" int oskar_amd_dipole_enu_launch(int precision, int matrix,
+        int offset_in, int num, const void* x, const void* y, const void* z,
+        int extra_point_at_pole, double delta_phi1, double delta_phi2,
+        double kL, double cos_kL, int swap_xy, int offset_out, void* pattern,
+        void* stream)
+{
+    const int total = num + (extra_point_at_pole ? 1 : 0);
+    if (total <= 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+#define DE(FP, M) hipLaunchKernelGGL((dipole_enu_kernel<FP, M>), \
+        grid_for(total), dim3(THREADS), 0, s, offset_in, num, (const FP*) x, \
+        (const FP*) y, (const FP*) z, extra_point_at_pole, (FP) delta_phi1, \
+        (FP) delta_phi2, (FP) kL, (FP) cos_kL, swap_xy, offset_out, \
+        (Cplx<FP>*) pattern)
+    if (precision == OSKAR_DOUBLE)
+    {
+        if (matrix) DE(double, true);
+        else DE(double, false);
+    }
+    else
+    {
+        if (matrix) DE(float, true);
+        else DE(float, false);
+    }
+#undef DE
+    return (int) hipGetLastError();
+}
+
 extern "C" int oskar_amd_dipole_launch(int precision, int matrix,
         int num_points, const void* theta, const void* phi_x,
         const void* phi_y, double kL, double cos_kL, int swap_xy,
@@ -654,28 +766,38 @@ extern "C" int oskar_amd_mem_normalise_launch(int type, unsigned int offset,
     return (int) hipGetLastError();
 }
 
-extern "C" int oskar_amd_normalise_copy_launch(int type, unsigned int n,
-        const void* data, unsigned int idx, void* out, void* stream)
+extern "C" int oskar_amd_beam_finish_launch(int type, unsigned int n,
+        const void* data, unsigned int idx, int normalise, const void* mask,
+        void* out, void* stream)
 {
     if (n == 0) return 0;
     const hipStream_t s = (hipStream_t) stream;
-#define NC_CASE(CODE, FP, NR) case CODE: \
-        hipLaunchKernelGGL((normalise_copy_kernel<FP, NR>), grid_for(n), \
-                dim3(THREADS), 0, s, n, (const FP*) data, idx, (FP*) out); \
-        break;
+    /* 16-byte vectors when both arrays start on 16-byte boundaries (every
+     * element then does, for 16- and 32-byte elements). */
+    const bool v16 = (((uintptr_t) data | (uintptr_t) out) & 15u) == 0;
     switch (type)
     {
-    NC_CASE(OSKAR_SINGLE, float, 1)
-    NC_CASE(OSKAR_DOUBLE, double, 1)
-    NC_CASE(OSKAR_SINGLE_COMPLEX, float, 2)
-    NC_CASE(OSKAR_DOUBLE_COMPLEX, double, 2)
-    NC_CASE(OSKAR_SINGLE_COMPLEX_MATRIX, float, 8)
-    NC_CASE(OSKAR_DOUBLE_COMPLEX_MATRIX, double, 8)
+    case OSKAR_SINGLE_COMPLEX:
+        return beam_finish_t<float, 2, float2>(n, data, idx, normalise, mask,
+                out, s);
+    case OSKAR_DOUBLE_COMPLEX:
+        return v16 ? beam_finish_t<double, 2, double2>(n, data, idx,
+                normalise, mask, out, s) :
+                beam_finish_t<double, 2, double>(n, data, idx, normalise,
+                mask, out, s);
+    case OSKAR_SINGLE_COMPLEX_MATRIX:
+        return v16 ? beam_finish_t<float, 8, float4>(n, data, idx,
+                normalise, mask, out, s) :
+                beam_finish_t<float, 8, float2>(n, data, idx, normalise,
+                mask, out, s);
+    case OSKAR_DOUBLE_COMPLEX_MATRIX:
+        return v16 ? beam_finish_t<double, 8, double2>(n, data, idx,
+                normalise, mask, out, s) :
+                beam_finish_t<double, 8, double>(n, data, idx, normalise,
+                mask, out, s);
     default:
         return OSKAR_ERR_BAD_DATA_TYPE;
     }
-#undef NC_CASE
-    return (int) hipGetLastError();
 }
 
 extern "C" int oskar_amd_copy3_point_launch(int precision, unsigned int n,

@@ -229,6 +229,15 @@ void oskar_amd_check_launch(int* status);
 int oskar_amd_hip_status(int hip_error);
 void oskar_amd_set_last_error(int error);
 
+/* ---- station beam (station.c) -------------------------------------------
+ * oskar_evaluate_station_beam_aperture_array without the final horizon
+ * blank of the top-level station (child stations are still blanked), for
+ * a caller that blanks while it copies the beam out. */
+void oskar_amd_station_beam_aa_unblanked(oskar_Station* s,
+        oskar_StationWork* work, int num_points, const oskar_Mem* x,
+        const oskar_Mem* y, const oskar_Mem* z, int time_index,
+        double gast_rad, double frequency_hz, oskar_Mem* beam, int* status);
+
 /* ---- piecewise chunk I/O (binary.c) --------------------------------------
  * A chunk written in pieces: begin writes the tag (data_size must equal the
  * sum of the pieces), each piece extends the CRC, end writes it.  A chunk
@@ -333,8 +342,12 @@ int oskar_amd_dftw_launch(int precision, int matrix, int is_3d,
 /* beam_util.hip */
 int oskar_amd_set_point3_launch(int precision, void* x, void* y, void* z,
         size_t index, double vx, double vy, double vz, void* stream);
-int oskar_amd_normalise_copy_launch(int type, unsigned int n,
-        const void* data, unsigned int idx, void* out, void* stream);
+/* out[0 .. n) = data[0 .. n) with elements whose mask value is negative
+ * set to zero (mask may be null) and, when normalise is set, scaled as
+ * oskar_mem_normalise scales by element idx (blanked alike; n <= idx). */
+int oskar_amd_beam_finish_launch(int type, unsigned int n,
+        const void* data, unsigned int idx, int normalise, const void* mask,
+        void* out, void* stream);
 int oskar_amd_copy3_point_launch(int precision, unsigned int n,
         const void* x, const void* y, const void* z, void* xo, void* yo,
         void* zo, int extra, double px, double py, double pz, void* stream);
@@ -349,6 +362,14 @@ int oskar_amd_theta_phi_launch(int precision, int offset_in, int num,
 int oskar_amd_dipole_launch(int precision, int matrix, int num_points,
         const void* theta, const void* phi_x, const void* phi_y, double kL,
         double cos_kL, int swap_xy, int offset_out, void* pattern,
+        void* stream);
+/* The dipole pattern straight from ENU directions (theta_phi then dipole,
+ * theta and phi never stored): directions [offset_in, offset_in + num),
+ * plus the pole at index num with extra_point_at_pole. */
+int oskar_amd_dipole_enu_launch(int precision, int matrix, int offset_in,
+        int num, const void* x, const void* y, const void* z,
+        int extra_point_at_pole, double delta_phi1, double delta_phi2,
+        double kL, double cos_kL, int swap_xy, int offset_out, void* pattern,
         void* stream);
 int oskar_amd_blank_below_horizon_launch(int precision, int matrix,
         int offset_mask, int num, const void* mask, int offset_out,

@@ -1300,31 +1300,50 @@ void oskar_blank_below_horizon(int offset_mask, int num_sources,
 static void element_evaluate(const oskar_Element* model, int normalise,
         int swap_xy, double orientation_x, double orientation_y,
         int offset_points, int num_points, const oskar_Mem* x, const oskar_Mem* y,
-        const oskar_Mem* z, double frequency_hz, oskar_Mem* theta,
-        oskar_Mem* phi_x, oskar_Mem* phi_y, int offset_out, oskar_Mem* output,
-        int* status)
+        const oskar_Mem* z, double frequency_hz, int offset_out,
+        oskar_Mem* output, int* status)
 {
     double dipole_length_m = 0.0;
     if (*status) return;
     const int num_points_norm = normalise ? 1 + num_points : num_points;
     oskar_mem_ensure(output, (size_t) (num_points_norm + offset_out), status);
-    oskar_mem_ensure(theta, (size_t) num_points_norm, status);
-    oskar_mem_ensure(phi_x, (size_t) num_points_norm, status);
-    oskar_mem_ensure(phi_y, (size_t) num_points_norm, status);
     dipole_length_m = model->dipole_length;
     if (model->dipole_length_units == OSKAR_WAVELENGTHS)
         dipole_length_m *= (C_0 / frequency_hz);
-    oskar_convert_enu_directions_to_theta_phi(offset_points, num_points,
-            x, y, z,
-            normalise, (M_PI / 2) - orientation_x, (M_PI / 2) - orientation_y,
-            theta, phi_x, phi_y, status);
+    if (*status) return;
     if (model->element_type == OSKAR_ELEMENT_TYPE_ISOTROPIC)
         oskar_mem_set_value_real(output, 1.0, (size_t) offset_out,
                 (size_t) num_points_norm, status);
     else if (model->element_type == OSKAR_ELEMENT_TYPE_DIPOLE)
-        oskar_evaluate_dipole_pattern(num_points_norm, theta, phi_x, phi_y,
-                frequency_hz, dipole_length_m, swap_xy, offset_out, output,
-                status);
+    {
+        /* oskar_convert_enu_directions_to_theta_phi followed by
+         * oskar_evaluate_dipole_pattern, in one kernel (the reference's two
+         * steps, oskar_element_evaluate.c:46-80, with theta and phi kept
+         * in registers). */
+        const int precision = oskar_mem_precision(output);
+        const double kL = dipole_length_m * (M_PI * frequency_hz / C_0);
+        if (oskar_mem_type(x) != precision || oskar_mem_type(y) != precision ||
+                oskar_mem_type(z) != precision)
+        {
+            *status = OSKAR_ERR_TYPE_MISMATCH;
+            return;
+        }
+        if (!check_gpu(x, status) || !check_gpu(output, status)) return;
+        if (oskar_mem_length(x) < (size_t) (offset_points + num_points) ||
+                oskar_mem_length(y) < (size_t) (offset_points + num_points) ||
+                oskar_mem_length(z) < (size_t) (offset_points + num_points))
+        {
+            *status = OSKAR_ERR_DIMENSION_MISMATCH;
+            return;
+        }
+        *status = oskar_amd_hip_status(oskar_amd_dipole_enu_launch(precision,
+                oskar_mem_is_matrix(output), offset_points, num_points,
+                oskar_mem_void_const(x), oskar_mem_void_const(y),
+                oskar_mem_void_const(z), normalise,
+                (M_PI / 2) - orientation_x, (M_PI / 2) - orientation_y, kL,
+                cos(kL), swap_xy, offset_out, oskar_mem_void(output),
+                oskar_amd_current_stream()));
+    }
     else *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE;
     if (normalise)
         oskar_mem_normalise(output, (size_t) offset_out, (size_t) num_points,
@@ -1412,12 +1431,13 @@ static void station_weights(const oskar_Station* s, int feed,
 /* One station of a (possibly hierarchical) aperture array, for points
  * [offset_points, offset_points + num_points) of x, y, z, into beam from
  * offset_out (oskar_evaluate_station_beam_aperture_array.c:70-288, the
- * branches for element types and for child stations). */
+ * branches for element types and for child stations).  The points below
+ * the horizon are blanked at the end unless `blank` is 0. */
 static void aa_beam(oskar_Station* s, oskar_StationWork* work,
         int offset_points, int num_points, const oskar_Mem* x,
         const oskar_Mem* y, const oskar_Mem* z, int time_index,
         double gast_rad, double frequency_hz, int depth, int offset_out,
-        oskar_Mem* beam, int* status)
+        oskar_Mem* beam, int blank, int* status)
 {
     double beam_x = 0.0, beam_y = 0.0, beam_z = 0.0;
     oskar_Mem* signal = 0;
@@ -1469,7 +1489,7 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
                     status);
             aa_beam(s->child[0], work, offset_points, num_points, x, y, z,
                     time_index, gast_rad, frequency_hz, depth + 1, 0, signal,
-                    status);
+                    1, status);
             child_idx = work_zero_idx(work, s->num_elements, status);
         }
         else
@@ -1480,7 +1500,7 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
             for (i = 0; i < s->num_elements; ++i)
                 aa_beam(s->child[i], work, offset_points, num_points, x, y, z,
                         time_index, gast_rad, frequency_hz, depth + 1,
-                        i * num_points, signal, status);
+                        i * num_points, signal, 1, status);
         }
         for (feed = 0; feed < num_feeds; ++feed)
         {
@@ -1513,7 +1533,7 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
                     oskar_station_element_euler_index_rad(s, 0, 0, 0) +
                     M_PI / 2, oskar_station_element_euler_index_rad(s, 1, 0, 0),
                     offset_points, num_points, x, y, z, frequency_hz,
-                    work->theta, work->phi_x, work->phi_y, i * num_points,
+                    i * num_points,
                     signal, status);
         }
     }
@@ -1539,7 +1559,7 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
                     oskar_station_element_euler_index_rad(s, 0, 0, i) +
                     M_PI / 2, oskar_station_element_euler_index_rad(s, 1, 0, i),
                     offset_points, num_points, x, y, z, frequency_hz,
-                    work->theta, work->phi_x, work->phi_y, i * num_points,
+                    i * num_points,
                     signal, status);
         }
     }
@@ -1571,8 +1591,9 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
             return;
         }
     }
-    oskar_blank_below_horizon(offset_points, num_points, z, offset_out, beam,
-            status);
+    if (blank)
+        oskar_blank_below_horizon(offset_points, num_points, z, offset_out,
+                beam, status);
 }
 
 /* Points per pass through a hierarchical station: each depth holds one
@@ -1601,10 +1622,10 @@ static int max_children(const oskar_Station* s)
     return m;
 }
 
-void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
-        oskar_StationWork* work, int num_points, const oskar_Mem* x,
-        const oskar_Mem* y, const oskar_Mem* z, int time_index,
-        double gast_rad, double frequency_hz, oskar_Mem* beam, int* status)
+static void aa_entry(oskar_Station* s, oskar_StationWork* work,
+        int num_points, const oskar_Mem* x, const oskar_Mem* y,
+        const oskar_Mem* z, int time_index, double gast_rad,
+        double frequency_hz, oskar_Mem* beam, int blank, int* status)
 {
     int start = 0;
     if (*status) return;
@@ -1623,7 +1644,7 @@ void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
     if (!s->child)
     {
         aa_beam(s, work, 0, num_points, x, y, z, time_index, gast_rad,
-                frequency_hz, 0, 0, beam, status);
+                frequency_hz, 0, 0, beam, blank, status);
         return;
     }
     /* Depth 0 is the element level, so the recursion starts at 1 (:52-66). */
@@ -1638,6 +1659,24 @@ void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
     {
         const int n = (num_points - start < pass) ? num_points - start : pass;
         aa_beam(s, work, start, n, x, y, z, time_index, gast_rad,
-                frequency_hz, 1, start, beam, status);
+                frequency_hz, 1, start, beam, blank, status);
     }
+}
+
+void oskar_evaluate_station_beam_aperture_array(oskar_Station* s,
+        oskar_StationWork* work, int num_points, const oskar_Mem* x,
+        const oskar_Mem* y, const oskar_Mem* z, int time_index,
+        double gast_rad, double frequency_hz, oskar_Mem* beam, int* status)
+{
+    aa_entry(s, work, num_points, x, y, z, time_index, gast_rad,
+            frequency_hz, beam, 1, status);
+}
+
+void oskar_amd_station_beam_aa_unblanked(oskar_Station* s,
+        oskar_StationWork* work, int num_points, const oskar_Mem* x,
+        const oskar_Mem* y, const oskar_Mem* z, int time_index,
+        double gast_rad, double frequency_hz, oskar_Mem* beam, int* status)
+{
+    aa_entry(s, work, num_points, x, y, z, time_index, gast_rad,
+            frequency_hz, beam, 0, status);
 }

@@ -268,24 +268,29 @@ void oskar_station_beam(oskar_Station* station, oskar_StationWork* work,
     }
     oskar_mem_ensure(work->beam_out, (size_t) n, status);
     if (*status) return;
+    /* The horizon blank of a flat aperture-array station (a hierarchical
+     * station's top level has none: its children blank their beams) is
+     * done by the kernel that copies the beam out, with the normalisation
+     * by the beam at the normalisation point (the last one) as
+     * oskar_mem_normalise does it. */
+    const oskar_Mem* mask = 0;
     if (type == OSKAR_STATION_TYPE_ISOTROPIC)
         oskar_mem_set_value_real(work->beam_out, 1.0, 0, (size_t) n, status);
     else
-        oskar_evaluate_station_beam_aperture_array(station, work, n,
-                work->enu[0], work->enu[1], work->enu[2], time_index,
-                gast_rad, frequency_hz, work->beam_out, status);
-    if (normalise && !*status)
     {
-        /* Divided by the beam at the normalisation point (the last one)
-         * while copied to the output (oskar_mem_normalise's arithmetic). */
+        oskar_amd_station_beam_aa_unblanked(station, work, n, work->enu[0],
+                work->enu[1], work->enu[2], time_index, gast_rad,
+                frequency_hz, work->beam_out, status);
+        if (!station->child) mask = work->enu[2];
+    }
+    if (*status) return;
+    {
         const size_t es = oskar_mem_element_size(oskar_mem_type(beam));
-        *status = oskar_amd_hip_status(oskar_amd_normalise_copy_launch(
+        *status = oskar_amd_hip_status(oskar_amd_beam_finish_launch(
                 oskar_mem_type(beam), (unsigned int) num_points,
                 oskar_mem_void_const(work->beam_out), (unsigned int) (n - 1),
+                normalise, mask ? oskar_mem_void_const(mask) : 0,
                 (char*) oskar_mem_void(beam) + (size_t) offset_out * es,
                 oskar_amd_current_stream()));
     }
-    else
-        oskar_mem_copy_contents(beam, work->beam_out, (size_t) offset_out, 0,
-                (size_t) num_points, status);
 }
