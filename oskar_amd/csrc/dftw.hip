@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
+#include "dipole_device.h"
 #include "private.h"
 
 using namespace oskar_amd;
@@ -205,6 +206,187 @@ void dftw_kernel(const int num_in, const FP wavenumber,
     }
 }
 
+/* The beam of a flat aperture-array station whose elements are all of one
+ * dipole type with a common orientation, in one pass per direction: the
+ * array factor (dftw_kernel's sum, in its order), times the direction's
+ * dipole pattern (computed here with the pattern kernels' device
+ * functions instead of read from a stored pattern), times norm_factor,
+ * zero below the horizon (blank_kernel's rule).  With NORM the result is
+ * divided as oskar_mem_normalise divides by the beam in the normalisation
+ * direction (nx, ny, nz), which every workgroup forms itself: each thread
+ * sums the array factor over a stride of the elements, a fixed-order tree
+ * in LDS adds the 256 partial sums (so every workgroup gets the same
+ * value), and thread 0 applies the pattern, the blank and the scale rule.
+ * Only the order of that one sum differs from the separate kernels. */
+template <typename FP, bool MATRIX, bool IS_3D, bool NORM>
+__global__ __launch_bounds__(THREADS)
+void aa_fused_kernel(const int num_in, const FP wavenumber,
+        const Cplx<FP>* __restrict__ weights, const FP* __restrict__ x_in,
+        const FP* __restrict__ y_in, const FP* __restrict__ z_in,
+        const int num_out, const FP* __restrict__ x_out,
+        const FP* __restrict__ y_out, const FP* __restrict__ z_out,
+        const FP nx, const FP ny, const FP nz, const FP kL, const FP cos_kL,
+        const FP dphi_x, const FP dphi_y, const int swap_xy,
+        const FP norm_factor, Cplx<FP>* __restrict__ output)
+{
+    constexpr int NC = MATRIX ? 4 : 1;
+    __shared__ Elem<FP> c_e[CHUNK];
+    __shared__ Cplx<FP> red[THREADS];
+    __shared__ FP s_scal;
+    const int tid = (int) threadIdx.x;
+    const FP kk = wavenumber * phase_unit<FP>();
+
+    /* The array factor plus w exp(i t) of one element: four fused
+     * multiply-adds, as in dftw_kernel. */
+    auto add_term = [](Cplx<FP>& af, FP t, const Cplx<FP>& w) {
+        FP s, c;
+        phase_sincos<FP>(t, &s, &c);
+        af.x += w.x * c;
+        af.x -= w.y * s;
+        af.y += w.x * s;
+        af.y += w.y * c;
+    };
+    /* The beam of one direction from its array factor: pattern x af,
+     * scaled, blanked (dftw_kernel's flush and output, blank_kernel). */
+    auto beam_of = [&](FP x, FP y, FP z, const Cplx<FP>& af, FP* v) {
+        Cplx<FP> d[NC];
+        FP theta, p1, p2;
+        theta_phi_point<FP>(x, y, z, dphi_x, dphi_y, &theta, &p1, &p2);
+        dipole_point<FP, MATRIX>(theta, p1, p2, kL, cos_kL, swap_xy, d, 0);
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+        {
+            Cplx<FP> o;
+            o.x = o.y = (FP) 0;
+            o.x += d[k].x * af.x - d[k].y * af.y;
+            o.y += d[k].y * af.x + d[k].x * af.y;
+            v[2 * k] = z < (FP) 0 ? (FP) 0 : o.x * norm_factor;
+            v[2 * k + 1] = z < (FP) 0 ? (FP) 0 : o.y * norm_factor;
+        }
+    };
+
+    FP scal = (FP) 1;
+    if constexpr (NORM)
+    {
+        const FP xn = kk * nx, yn = kk * ny;
+        const FP zn = IS_3D ? kk * nz : (FP) 0;
+        Cplx<FP> a;
+        a.x = a.y = (FP) 0;
+        for (int e = tid; e < num_in; e += THREADS)
+        {
+            FP t = xn * x_in[e] + yn * y_in[e];
+            if (IS_3D) t += zn * z_in[e];
+            add_term(a, t, weights[e]);
+        }
+        red[tid] = a;
+        __syncthreads();
+        for (int o = THREADS / 2; o > 0; o >>= 1)
+        {
+            if (tid < o)
+            {
+                red[tid].x += red[tid + o].x;
+                red[tid].y += red[tid + o].y;
+            }
+            __syncthreads();
+        }
+        if (tid == 0)
+        {
+            FP v[2 * NC];
+            beam_of(nx, ny, nz, red[0], v);
+            FP amp = (FP) 0;
+#pragma unroll
+            for (int r = 0; r < 2 * NC; ++r) amp += v[r] * v[r];
+            if (MATRIX) amp /= (FP) 2;
+            s_scal = (FP) 1 / sqrt(amp);
+        }
+        __syncthreads();
+        scal = s_scal;
+    }
+
+    const int i_out = blockIdx.x * THREADS + tid;
+    const bool active = i_out < num_out;
+    FP xd = (FP) 0, yd = (FP) 0, zd = (FP) 0;
+    if (active)
+    {
+        xd = x_out[i_out];
+        yd = y_out[i_out];
+        zd = z_out[i_out];
+    }
+    const FP xo = kk * xd, yo = kk * yd, zo = IS_3D ? kk * zd : (FP) 0;
+    Cplx<FP> af;
+    af.x = af.y = (FP) 0;
+    auto term = [&](const Elem<FP>& e) {
+        FP t = xo * e.x + yo * e.y;
+        if (IS_3D) t += zo * e.z;
+        add_term(af, t, e.w);
+    };
+    for (int j = 0; j < num_in; j += CHUNK)
+    {
+        const int chunk = min(CHUNK, num_in - j);
+        __syncthreads();
+        for (int t = tid; t < chunk; t += THREADS)
+        {
+            const int g = j + t;
+            Elem<FP> e;
+            e.x = x_in[g];
+            e.y = y_in[g];
+            e.z = IS_3D ? z_in[g] : (FP) 0;
+            e.pad = (FP) 0;
+            e.w = weights[g];
+            c_e[t] = e;
+        }
+        __syncthreads();
+        int i = 0;
+        for (; i + 4 <= chunk; i += 4)
+        {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) term(c_e[i + u]);
+        }
+        for (; i < chunk; ++i) term(c_e[i]);
+    }
+    if (!active) return;
+    FP v[2 * NC];
+    beam_of(xd, yd, zd, af, v);
+    Cplx<FP>* o = output + (size_t) i_out * NC;
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+    {
+        Cplx<FP> r;
+        r.x = v[2 * k];
+        r.y = v[2 * k + 1];
+        if constexpr (NORM)
+        {
+            r.x = r.x * scal;
+            r.y = r.y * scal;
+        }
+        o[k] = r;
+    }
+}
+
+template <typename FP, bool MATRIX, bool IS_3D>
+hipError_t aa_fused_t(int normalise, int num_in, double wavenumber,
+        const void* weights, const void* x_in, const void* y_in,
+        const void* z_in, int num_out, const void* x_out, const void* y_out,
+        const void* z_out, double nx, double ny, double nz, double kL,
+        double cos_kL, double dphi_x, double dphi_y, int swap_xy,
+        double norm_factor, void* output, hipStream_t stream)
+{
+    const dim3 grid((num_out + THREADS - 1) / THREADS);
+#define AF_ARGS num_in, (FP) wavenumber, (const Cplx<FP>*) weights,       \
+        (const FP*) x_in, (const FP*) y_in, (const FP*) z_in, num_out,      \
+        (const FP*) x_out, (const FP*) y_out, (const FP*) z_out, (FP) nx,   \
+        (FP) ny, (FP) nz, (FP) kL, (FP) cos_kL, (FP) dphi_x, (FP) dphi_y,   \
+        swap_xy, (FP) norm_factor, (Cplx<FP>*) output
+    if (normalise)
+        hipLaunchKernelGGL((aa_fused_kernel<FP, MATRIX, IS_3D, true>), grid,
+                dim3(THREADS), 0, stream, AF_ARGS);
+    else
+        hipLaunchKernelGGL((aa_fused_kernel<FP, MATRIX, IS_3D, false>), grid,
+                dim3(THREADS), 0, stream, AF_ARGS);
+#undef AF_ARGS
+    return hipGetLastError();
+}
+
 template <typename FP>
 hipError_t launch_fp(int matrix, int is_3d, int num_in, double wavenumber,
         const void* weights, const void* x_in, const void* y_in,
@@ -257,4 +439,30 @@ extern "C" int oskar_amd_dftw_launch(int precision, int matrix, int is_3d,
             weights, x_in, y_in, z_in, offset_coord_out, num_out, x_out,
             y_out, z_out, data_idx, data, num_rows, eval_x, eval_y,
             offset_out, output, norm_factor, (hipStream_t) stream);
+}
+
+extern "C" int oskar_amd_aa_fused_launch(int precision, int matrix,
+        int is_3d, int normalise, int num_in, double wavenumber,
+        const void* weights, const void* x_in, const void* y_in,
+        const void* z_in, int num_out, const void* x_out, const void* y_out,
+        const void* z_out, double nx, double ny, double nz, double kL,
+        double cos_kL, double dphi_x, double dphi_y, int swap_xy,
+        double norm_factor, void* output, void* stream)
+{
+    if (num_out <= 0) return 0;
+    const hipStream_t s = (hipStream_t) stream;
+#define AF(FP, M, D) return (int) aa_fused_t<FP, M, D>(normalise, num_in, \
+        wavenumber, weights, x_in, y_in, z_in, num_out, x_out, y_out, z_out, \
+        nx, ny, nz, kL, cos_kL, dphi_x, dphi_y, swap_xy, norm_factor, output, \
+        s)
+    if (precision == OSKAR_DOUBLE)
+    {
+        if (matrix) { if (is_3d) AF(double, true, true); AF(double, true, false); }
+        if (is_3d) AF(double, false, true);
+        AF(double, false, false);
+    }
+    if (matrix) { if (is_3d) AF(float, true, true); AF(float, true, false); }
+    if (is_3d) AF(float, false, true);
+    AF(float, false, false);
+#undef AF
 }

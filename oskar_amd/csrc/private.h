@@ -238,6 +238,21 @@ void oskar_amd_station_beam_aa_unblanked(oskar_Station* s,
         const oskar_Mem* y, const oskar_Mem* z, int time_index,
         double gast_rad, double frequency_hz, oskar_Mem* beam, int* status);
 
+/* The beam of a flat aperture-array station whose elements are all of one
+ * dipole type with a common orientation and common polarisation beams, in
+ * one kernel per call (dftw.hip aa_fused_kernel): array factor x dipole
+ * pattern, blanked below the horizon and, with normalise, divided as
+ * oskar_mem_normalise divides by the beam in the direction (nx, ny, nz).
+ * covers() says whether a station and beam type are of that kind; the
+ * beam goes to beam from offset_out. */
+int oskar_amd_station_beam_aa_fused_covers(const oskar_Station* s,
+        const oskar_Mem* beam);
+void oskar_amd_station_beam_aa_fused(oskar_Station* s,
+        oskar_StationWork* work, int num_points, const oskar_Mem* x,
+        const oskar_Mem* y, const oskar_Mem* z, int normalise, double nx,
+        double ny, double nz, int time_index, double gast_rad,
+        double frequency_hz, int offset_out, oskar_Mem* beam, int* status);
+
 /* ---- piecewise chunk I/O (binary.c) --------------------------------------
  * A chunk written in pieces: begin writes the tag (data_size must equal the
  * sum of the pieces), each piece extends the CRC, end writes it.  A chunk
@@ -362,6 +377,15 @@ int oskar_amd_theta_phi_launch(int precision, int offset_in, int num,
 int oskar_amd_dipole_launch(int precision, int matrix, int num_points,
         const void* theta, const void* phi_x, const void* phi_y, double kL,
         double cos_kL, int swap_xy, int offset_out, void* pattern,
+        void* stream);
+/* dftw.hip: the fused aperture-array beam (see
+ * oskar_amd_station_beam_aa_fused); output already offset. */
+int oskar_amd_aa_fused_launch(int precision, int matrix, int is_3d,
+        int normalise, int num_in, double wavenumber, const void* weights,
+        const void* x_in, const void* y_in, const void* z_in, int num_out,
+        const void* x_out, const void* y_out, const void* z_out, double nx,
+        double ny, double nz, double kL, double cos_kL, double dphi_x,
+        double dphi_y, int swap_xy, double norm_factor, void* output,
         void* stream);
 /* The dipole pattern straight from ENU directions (theta_phi then dipole,
  * theta and phi never stored): directions [offset_in, offset_in + num),

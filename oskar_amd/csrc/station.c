@@ -1596,6 +1596,101 @@ static void aa_beam(oskar_Station* s, oskar_StationWork* work,
                 beam, status);
 }
 
+int oskar_amd_station_beam_aa_fused_covers(const oskar_Station* s,
+        const oskar_Mem* beam)
+{
+    int i = 0;
+    if (!s || s->station_type != OSKAR_STATION_TYPE_AA || s->child ||
+            s->mem_location != OSKAR_GPU ||
+            !s->common_element_orientation || s->num_element_types != 1 ||
+            !s->enable_array_pattern || s->normalise_element_pattern ||
+            s->num_elements <= 0 || !s->element || !s->element[0] ||
+            s->element[0]->element_type != OSKAR_ELEMENT_TYPE_DIPOLE ||
+            s->precision != oskar_mem_precision(beam) ||
+            !oskar_mem_is_complex(beam))
+        return 0;
+    /* One feed beam for both rows of a matrix (:89-90). */
+    if (!s->common_pol_beams && oskar_mem_is_matrix(beam)) return 0;
+    /* Every element's data row is the one pattern. */
+    if (!s->element_types_cpu ||
+            (int) oskar_mem_length(s->element_types_cpu) < s->num_elements)
+        return 0;
+    {
+        const int* type = (const int*) oskar_mem_void_const(
+                s->element_types_cpu);
+        for (i = 0; i < s->num_elements; ++i)
+            if (type[i] != 0) return 0;
+    }
+    return 1;
+}
+
+void oskar_amd_station_beam_aa_fused(oskar_Station* s,
+        oskar_StationWork* work, int num_points, const oskar_Mem* x,
+        const oskar_Mem* y, const oskar_Mem* z, int normalise, double nx,
+        double ny, double nz, int time_index, double gast_rad,
+        double frequency_hz, int offset_out, oskar_Mem* beam, int* status)
+{
+    double beam_x = 0.0, beam_y = 0.0, beam_z = 0.0;
+    int i = 0;
+    if (*status) return;
+    if (!oskar_amd_station_beam_aa_fused_covers(s, beam))
+    {
+        *status = OSKAR_ERR_FUNCTION_NOT_AVAILABLE;
+        return;
+    }
+    for (i = 0; i < 3; ++i)
+    {
+        const oskar_Mem* d = i == 0 ? x : (i == 1 ? y : z);
+        if (oskar_mem_type(d) != s->precision)
+        {
+            *status = OSKAR_ERR_TYPE_MISMATCH;
+            return;
+        }
+        if (!check_gpu(d, status)) return;
+        if (oskar_mem_length(d) < (size_t) num_points)
+        {
+            *status = OSKAR_ERR_DIMENSION_MISMATCH;
+            return;
+        }
+    }
+    const double wavenumber = 2.0 * M_PI * frequency_hz / C_0;
+    oskar_station_beam_horizon_direction(s, gast_rad, &beam_x, &beam_y,
+            &beam_z, status);
+    oskar_mem_ensure(beam, (size_t) offset_out + (size_t) num_points, status);
+    station_weights(s, 0, work, wavenumber, beam_x, beam_y, beam_z,
+            time_index, status);
+    if (*status || num_points == 0) return;
+    {
+        /* The element pattern's parameters as element_evaluate forms them
+         * for oskar_convert_enu_directions_to_theta_phi and
+         * oskar_evaluate_dipole_pattern. */
+        const oskar_Element* model = s->element[0];
+        double dipole_length_m = model->dipole_length;
+        if (model->dipole_length_units == OSKAR_WAVELENGTHS)
+            dipole_length_m *= (C_0 / frequency_hz);
+        const double kL = dipole_length_m * (M_PI * frequency_hz / C_0);
+        const double orientation_x =
+                oskar_station_element_euler_index_rad(s, 0, 0, 0) + M_PI / 2;
+        const double orientation_y =
+                oskar_station_element_euler_index_rad(s, 1, 0, 0);
+        const size_t es = oskar_mem_element_size(oskar_mem_type(beam));
+        *status = oskar_amd_hip_status(oskar_amd_aa_fused_launch(
+                s->precision, oskar_mem_is_matrix(beam), s->array_is_3d,
+                normalise, s->num_elements, wavenumber,
+                oskar_mem_void_const(work->weights),
+                oskar_mem_void_const(feed_mem(s, 0, F_TRUE)),
+                oskar_mem_void_const(feed_mem(s, 0, F_TRUE + 1)),
+                oskar_mem_void_const(feed_mem(s, 0, F_TRUE + 2)),
+                num_points, oskar_mem_void_const(x), oskar_mem_void_const(y),
+                oskar_mem_void_const(z), nx, ny, nz, kL, cos(kL),
+                (M_PI / 2) - orientation_x, (M_PI / 2) - orientation_y,
+                s->swap_xy,
+                s->normalise_array_pattern ? 1.0 / s->num_elements : 1.0,
+                (char*) oskar_mem_void(beam) + (size_t) offset_out * es,
+                oskar_amd_current_stream()));
+    }
+}
+
 /* Points per pass through a hierarchical station: each depth holds one
  * beam per child for every point of the pass (the reference's
  * MAX_CHUNK_SIZE, oskar_evaluate_station_beam_aperture_array.c:21, is 49152;

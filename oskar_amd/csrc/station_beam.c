@@ -224,6 +224,28 @@ void oskar_station_beam(oskar_Station* station, oskar_StationWork* work,
                 oskar_mem_type(source_coords[i]) ==
                 oskar_mem_type(work->enu[i]) &&
                 oskar_mem_length(source_coords[i]) >= (size_t) n;
+    if (*status) return;
+    if (type == OSKAR_STATION_TYPE_AA &&
+            oskar_amd_station_beam_aa_fused_covers(station, beam))
+    {
+        /* A flat station of one dipole type: one kernel forms the array
+         * factor, the element pattern, the blank and the normalisation and
+         * writes beam (directions read where they are when they are ENU
+         * on the device, with no normalisation point appended). */
+        const oskar_Mem* d[3];
+        for (i = 0; i < 3; ++i) d[i] = source_coords[i];
+        if (!enu_on_device)
+        {
+            oskar_convert_any_to_enu_directions(source_coord_type,
+                    num_points, source_coords, ref_lon_rad, ref_lat_rad,
+                    lst_rad, lat_rad, work->enu, status);
+            for (i = 0; i < 3; ++i) d[i] = work->enu[i];
+        }
+        oskar_amd_station_beam_aa_fused(station, work, num_points, d[0],
+                d[1], d[2], normalise, nx, ny, nz, time_index, gast_rad,
+                frequency_hz, offset_out, beam, status);
+        return;
+    }
     if (enu_on_device)
     {
         for (i = 0; i < 3; ++i)

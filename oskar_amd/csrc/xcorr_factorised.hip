@@ -349,6 +349,12 @@ struct Builder
         pg = bt / TP;
         qb = bt % TQ;
         qg = bt / TQ;
+        /* A group of at least 64 threads is whole waves: the group index
+         * is wave-uniform, and so are the source addresses of a k-step,
+         * which are then formed in scalar registers (14 fewer VALU
+         * instructions per k-step at 64 x 64). */
+        if constexpr (TP % 64 == 0) pg = __builtin_amdgcn_readfirstlane(pg);
+        if constexpr (TQ % 64 == 0) qg = __builtin_amdgcn_readfirstlane(qg);
         ps = min(J.p0 + pb, N - 1);
         qs = min(J.q0 + qb, N - 1);
         pu = gld(J.st_uvw + 3 * ps);

@@ -1,5 +1,9 @@
-"""GPU tests of the last step of oskar_station_beam: the horizon blank and
-the normalisation, which one kernel applies while it copies the beam out.
+"""GPU tests of the last steps of oskar_station_beam: the horizon blank and
+the normalisation.  For a flat station of one dipole type one kernel forms
+the whole beam with them (array factor, element pattern, blank, division
+by the beam in the normalisation direction, whose array factor it sums in
+a different order); other stations go through the separate steps and one
+kernel blanks and normalises while it copies the beam out.
 
 The reference blanks the aperture-array beam below the horizon
 (oskar_evaluate_station_beam_aperture_array.c:238-239, flat stations only)
@@ -10,7 +14,8 @@ the same directions with the normalisation direction appended, blanked by
 that function, then normalised in numpy:
   - flat stations, FP64 and FP32, matrix and scalar beams, normalisation on
     and off, about a third of the directions below the horizon (their rows
-    must be exact zeros), with an output offset;
+    must be exact zeros), with an output offset; elements in a plane and
+    (FP32 and FP64 matrix) with heights, a 3D array;
   - a hierarchical station (alike children), whose top level is not
     blanked (its children are), normalisation on.
 """
@@ -37,11 +42,12 @@ def azel_dir(az, el):
                      np.sin(el)])
 
 
-def flat_station(P, rng, ne=40):
+def flat_station(P, rng, ne=40, height=0.0):
     st = Station(P, OSKAR_GPU, ne)
     r = 12.0 * np.sqrt(rng.uniform(size=ne))
     t = rng.uniform(0, 2 * np.pi, ne)
-    xt = np.stack([r * np.cos(t), r * np.sin(t), np.zeros(ne)])
+    xt = np.stack([r * np.cos(t), r * np.sin(t),
+                   height * rng.uniform(-1.0, 1.0, ne)])
     st.set_coords(xt, xt)
     s = C.c_int(0)
     lib.oskar_station_resize_element_types(st.ptr, 1, C.byref(s))
@@ -100,16 +106,19 @@ def normalised(v, ref):
     return v / np.sqrt(amp)
 
 
-@pytest.mark.parametrize("prec", ["d", "s"])
-@pytest.mark.parametrize("matrix", [True, False])
-@pytest.mark.parametrize("normalise", [1, 0])
-def test_blank_and_normalise_flat_station(hip, prec, matrix, normalise):
-    print("station beam finish: %s matrix=%d normalise=%d"
-          % (prec, matrix, normalise), flush=True)
+CASES = [(p, m, nrm, 0.0) for p in ("d", "s") for m in (True, False)
+         for nrm in (1, 0)] + [("d", True, 1, 0.4), ("s", True, 1, 0.4)]
+
+
+@pytest.mark.parametrize("prec,matrix,normalise,height", CASES)
+def test_blank_and_normalise_flat_station(hip, prec, matrix, normalise,
+                                          height):
+    print("station beam finish: %s matrix=%d normalise=%d height=%g"
+          % (prec, matrix, normalise, height), flush=True)
     P = OSKAR_DOUBLE if prec == "d" else OSKAR_SINGLE
     f = np.float64 if prec == "d" else np.float32
     rng = np.random.default_rng(70 + 2 * matrix + normalise)
-    st = flat_station(P, rng)
+    st = flat_station(P, rng, height=height)
     lib.oskar_station_set_normalise_final_beam(st.ptr, normalise)
     work = StationWork(P, OSKAR_GPU)
     n, off = 3001, 5
