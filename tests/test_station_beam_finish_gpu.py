@@ -16,6 +16,8 @@ that function, then normalised in numpy:
     and off, about a third of the directions below the horizon (their rows
     must be exact zeros), with an output offset; elements in a plane and
     (FP32 and FP64 matrix) with heights, a 3D array;
+  - element gains and phases with time-variable random errors and
+    apodisation weights, at a time index other than 0;
   - a hierarchical station (alike children), whose top level is not
     blanked (its children are), normalisation on.
 """
@@ -67,7 +69,7 @@ def directions(rng, n):
     return v
 
 
-def aa_beam(st, work, P, matrix, x, y, z):
+def aa_beam(st, work, P, matrix, x, y, z, time_index=0):
     """The aperture-array beam of the given directions (blanked by the
     function itself)."""
     n = x.size
@@ -76,14 +78,14 @@ def aa_beam(st, work, P, matrix, x, y, z):
     out = Mem(t, OSKAR_GPU, n)
     s = C.c_int(0)
     lib.oskar_evaluate_station_beam_aperture_array(
-        st.ptr, work.ptr, n, ins[0].ptr, ins[1].ptr, ins[2].ptr, 0,
+        st.ptr, work.ptr, n, ins[0].ptr, ins[1].ptr, ins[2].ptr, time_index,
         lib.oskar_convert_mjd_to_gast_fast(MJD), FREQ, out.ptr, C.byref(s))
     check_status(s, "aa beam")
     sync_check("aa beam")
     return out.numpy().reshape(n, 8 if matrix else 2)
 
 
-def station_beam(st, work, P, matrix, x, y, z, offset_out):
+def station_beam(st, work, P, matrix, x, y, z, offset_out, time_index=0):
     n = x.size
     ins = [Mem.from_numpy(v, P) for v in (x, y, z)]
     t = P | OSKAR_COMPLEX | (OSKAR_MATRIX if matrix else 0)
@@ -91,8 +93,8 @@ def station_beam(st, work, P, matrix, x, y, z, offset_out):
     s = C.c_int(0)
     lib.oskar_station_beam(
         st.ptr, work.ptr, ENU_DIR, n, capi.MEM3(*[m.ptr for m in ins]), 0.0,
-        0.0, AZEL, float(AZ), float(EL), 0, MJD, MJD, FREQ, offset_out,
-        out.ptr, C.byref(s))
+        0.0, AZEL, float(AZ), float(EL), time_index, MJD, MJD, FREQ,
+        offset_out, out.ptr, C.byref(s))
     check_status(s, "oskar_station_beam")
     sync_check("oskar_station_beam")
     return out.numpy().reshape(n + offset_out, 8 if matrix else 2)
@@ -133,6 +135,44 @@ def test_blank_and_normalise_flat_station(hip, prec, matrix, normalise,
     want = normalised(aa[:n], aa[n]) if normalise else aa[:n]
     tol = 1e-12 if prec == "d" else 2e-6
     assert max_rel(got[~below], want[~below]) < tol
+
+
+@pytest.mark.parametrize("prec", ["d", "s"])
+def test_element_errors_and_weights(hip, prec):
+    print("station beam finish: element errors, %s" % prec, flush=True)
+    P = OSKAR_DOUBLE if prec == "d" else OSKAR_SINGLE
+    f = np.float64 if prec == "d" else np.float32
+    rng = np.random.default_rng(79)
+    ne = 40
+    st = flat_station(P, rng, ne=ne)
+    s = C.c_int(0)
+    for i in range(ne):
+        lib.oskar_station_set_element_errors(
+            st.ptr, 0, i, rng.uniform(0.8, 1.2), rng.uniform(0.0, 0.1),
+            rng.uniform(-10, 10), rng.uniform(0.0, 5.0), C.byref(s))
+        a = rng.uniform(0.3, 1.0) * np.exp(1j * rng.uniform(-0.3, 0.3))
+        lib.oskar_station_set_element_weight(st.ptr, 0, i, a.real, a.imag,
+                                             C.byref(s))
+    lib.oskar_station_set_seed_time_variable_errors(st.ptr, 9)
+    fin = C.c_int(0)
+    lib.oskar_station_analyse(st.ptr, C.byref(fin), C.byref(s))
+    check_status(s, "errors and weights")
+    assert lib.oskar_station_apply_element_errors(st.ptr) == 1
+    assert lib.oskar_station_apply_element_weight(st.ptr) == 1
+    lib.oskar_station_set_normalise_final_beam(st.ptr, 1)
+    work = StationWork(P, OSKAR_GPU)
+    n = 2000
+    x, y, z = (v.astype(f) for v in directions(rng, n))
+    bd = azel_dir(AZ, EL).astype(f)
+    aa = aa_beam(st, work, P, True, *(np.append(v, b) for v, b in
+                                      zip((x, y, z), bd)), time_index=5)
+    got = station_beam(st, work, P, True, x, y, z, 0, time_index=5)
+    assert not np.any(got[z < 0])
+    tol = 1e-12 if prec == "d" else 2e-6
+    assert max_rel(got, normalised(aa[:n], aa[n])) < tol
+    # Another time index draws other errors: the beam changes.
+    other = station_beam(st, work, P, True, x, y, z, 0, time_index=6)
+    assert max_rel(other, got) > 1e-4
 
 
 def test_normalise_hierarchical_station(hip):
