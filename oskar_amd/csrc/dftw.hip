@@ -28,6 +28,7 @@
  *    factor.  Double precision uses radians and the full-accuracy sincos.
  */
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "device_common.h"
 #include "dipole_device.h"
@@ -41,20 +42,25 @@ constexpr int THREADS = 256;
 constexpr int CHUNK = 256;
 
 /* exp(i t) of the phase in the unit the kernel carries it in: FP32 phases
- * are in revolutions, reduced to [-1/2, 1/2] and fed to the hardware sine
- * and cosine (which take revolutions); FP64 phases are in radians, with the
- * full-accuracy sincos. */
-template <typename FP> __device__ __forceinline__ void phase_sincos(FP t,
-        FP* s, FP* c);
-template <> __device__ __forceinline__ void phase_sincos<float>(float t,
-        float* s, float* c)
+ * are in revolutions for the hardware sine and cosine (which take
+ * revolutions), reduced to [-1/2, 1/2] first unless DIRECT; FP64 phases
+ * are in radians, with the full-accuracy sincos.  The hardware reduces a
+ * phase of up to 256 revolutions exactly (the same errors against the FP64
+ * DFT with and without the reduction at up to 250 revolutions, measured),
+ * so DIRECT serves when |t| is bounded below that: two fewer instructions
+ * per term. */
+template <bool DIRECT, typename FP>
+__device__ __forceinline__ void phase_sincos(FP t, FP* s, FP* c)
 {
-    const float r = t - __builtin_rintf(t);
-    *s = __builtin_amdgcn_sinf(r);
-    *c = __builtin_amdgcn_cosf(r);
+    if constexpr (sizeof(FP) == 8)
+        sincos(t, s, c);
+    else
+    {
+        const float r = DIRECT ? t : t - __builtin_rintf(t);
+        *s = __builtin_amdgcn_sinf(r);
+        *c = __builtin_amdgcn_cosf(r);
+    }
 }
-template <> __device__ __forceinline__ void phase_sincos<double>(double t,
-        double* s, double* c) { sincos(t, s, c); }
 
 /* Factor from k (x_out x_in + ...) to the kernel's phase unit. */
 template <typename FP> __device__ __forceinline__ FP phase_unit();
@@ -69,6 +75,32 @@ template <typename FP> struct Elem
     FP x, y, z, pad;
     Cplx<FP> w;
 };
+
+/* Largest |coordinate| of the chunk's elements, gathered in LDS as float
+ * bits (non-negative floats order as unsigned integers). */
+template <typename FP>
+__device__ __forceinline__ void chunk_extent(unsigned int* bits,
+        const Elem<FP>& e)
+{
+    if constexpr (sizeof(FP) == 4)
+    {
+        const float a = fmaxf(fabsf(e.x), fmaxf(fabsf(e.y), fabsf(e.z)));
+        atomicMax(bits, __float_as_uint(a));
+    }
+}
+
+/* Whether every phase of a thread's direction (xo, yo, zo, in revolutions
+ * per metre) over elements of extent at most ext stays within the
+ * hardware's exact reduction: |t| <= (|xo| + |yo| + |zo|) ext < 255. */
+template <typename FP>
+__device__ __forceinline__ bool direct_phase(FP xo, FP yo, FP zo,
+        unsigned int ext_bits)
+{
+    if constexpr (sizeof(FP) == 8) return false;
+    else
+        return (fabsf(xo) + fabsf(yo) + fabsf(zo)) *
+                __uint_as_float(ext_bits) < 255.0f;
+}
 
 template <typename FP, bool MATRIX, bool IS_3D>
 __global__ __launch_bounds__(THREADS)
@@ -86,6 +118,7 @@ void dftw_kernel(const int num_in, const FP wavenumber,
     __shared__ Elem<FP> c_e[CHUNK];
     __shared__ int c_idx[CHUNK];
     __shared__ int c_uniform;
+    __shared__ unsigned int c_ext;
 
     const int i_out = blockIdx.x * THREADS + threadIdx.x;
     const bool active = i_out < num_out;
@@ -123,22 +156,28 @@ void dftw_kernel(const int num_in, const FP wavenumber,
         af.x = af.y = (FP) 0;
     };
     /* af += w exp(i t) for element e: four fused multiply-adds. */
-    auto term = [&](const Elem<FP>& e) {
+    auto term = [&](const Elem<FP>& e, auto direct) {
         FP t = xo * e.x + yo * e.y;
         if (IS_3D) t += zo * e.z;
         FP s, c;
-        phase_sincos<FP>(t, &s, &c);
+        phase_sincos<decltype(direct)::value, FP>(t, &s, &c);
         af.x += e.w.x * c;
         af.x -= e.w.y * s;
         af.y += e.w.x * s;
         af.y += e.w.y * c;
     };
+    const std::true_type direct_t;
+    const std::false_type reduce_t;
 
     for (int j = 0; j < num_in; j += CHUNK)
     {
         const int chunk = min(CHUNK, num_in - j);
         __syncthreads();
-        if (threadIdx.x == 0) c_uniform = 1;
+        if (threadIdx.x == 0)
+        {
+            c_uniform = 1;
+            c_ext = 0u;
+        }
         __syncthreads();
         for (int t = threadIdx.x; t < chunk; t += THREADS)
         {
@@ -150,6 +189,7 @@ void dftw_kernel(const int num_in, const FP wavenumber,
             e.pad = (FP) 0;
             e.w = weights[g];
             c_e[t] = e;
+            chunk_extent<FP>(&c_ext, e);
             const int idx = data_idx ? data_idx[g] : g;
             c_idx[t] = idx;
             /* Any element whose data row differs from the chunk's first
@@ -157,6 +197,7 @@ void dftw_kernel(const int num_in, const FP wavenumber,
             if (idx != (data_idx ? data_idx[j] : j)) c_uniform = 0;
         }
         __syncthreads();
+        const bool direct = direct_phase<FP>(xo, yo, zo, c_ext);
         if (c_uniform)
         {
             /* One data row for the whole chunk: no run checks, and the
@@ -168,12 +209,24 @@ void dftw_kernel(const int num_in, const FP wavenumber,
                 run_idx = idx;
             }
             int i = 0;
-            for (; i + 4 <= chunk; i += 4)
+            if (direct)
             {
+                for (; i + 4 <= chunk; i += 4)
+                {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) term(c_e[i + u]);
+                    for (int u = 0; u < 4; ++u) term(c_e[i + u], direct_t);
+                }
+                for (; i < chunk; ++i) term(c_e[i], direct_t);
             }
-            for (; i < chunk; ++i) term(c_e[i]);
+            else
+            {
+                for (; i + 4 <= chunk; i += 4)
+                {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) term(c_e[i + u], reduce_t);
+                }
+                for (; i < chunk; ++i) term(c_e[i], reduce_t);
+            }
         }
         else
         {
@@ -185,7 +238,8 @@ void dftw_kernel(const int num_in, const FP wavenumber,
                     flush();
                     run_idx = idx;
                 }
-                term(c_e[i]);
+                if (direct) term(c_e[i], direct_t);
+                else term(c_e[i], reduce_t);
             }
         }
     }
@@ -233,14 +287,15 @@ void aa_fused_kernel(const int num_in, const FP wavenumber,
     __shared__ Elem<FP> c_e[CHUNK];
     __shared__ Cplx<FP> red[THREADS];
     __shared__ FP s_scal;
+    __shared__ unsigned int c_ext;
     const int tid = (int) threadIdx.x;
     const FP kk = wavenumber * phase_unit<FP>();
 
     /* The array factor plus w exp(i t) of one element: four fused
      * multiply-adds, as in dftw_kernel. */
-    auto add_term = [](Cplx<FP>& af, FP t, const Cplx<FP>& w) {
+    auto add_term = [](Cplx<FP>& af, FP t, const Cplx<FP>& w, auto direct) {
         FP s, c;
-        phase_sincos<FP>(t, &s, &c);
+        phase_sincos<decltype(direct)::value, FP>(t, &s, &c);
         af.x += w.x * c;
         af.x -= w.y * s;
         af.y += w.x * s;
@@ -276,7 +331,7 @@ void aa_fused_kernel(const int num_in, const FP wavenumber,
         {
             FP t = xn * x_in[e] + yn * y_in[e];
             if (IS_3D) t += zn * z_in[e];
-            add_term(a, t, weights[e]);
+            add_term(a, t, weights[e], std::false_type());
         }
         red[tid] = a;
         __syncthreads();
@@ -315,14 +370,18 @@ void aa_fused_kernel(const int num_in, const FP wavenumber,
     const FP xo = kk * xd, yo = kk * yd, zo = IS_3D ? kk * zd : (FP) 0;
     Cplx<FP> af;
     af.x = af.y = (FP) 0;
-    auto term = [&](const Elem<FP>& e) {
+    auto term = [&](const Elem<FP>& e, auto direct) {
         FP t = xo * e.x + yo * e.y;
         if (IS_3D) t += zo * e.z;
-        add_term(af, t, e.w);
+        add_term(af, t, e.w, direct);
     };
+    const std::true_type direct_t;
+    const std::false_type reduce_t;
     for (int j = 0; j < num_in; j += CHUNK)
     {
         const int chunk = min(CHUNK, num_in - j);
+        __syncthreads();
+        if (tid == 0) c_ext = 0u;
         __syncthreads();
         for (int t = tid; t < chunk; t += THREADS)
         {
@@ -334,15 +393,28 @@ void aa_fused_kernel(const int num_in, const FP wavenumber,
             e.pad = (FP) 0;
             e.w = weights[g];
             c_e[t] = e;
+            chunk_extent<FP>(&c_ext, e);
         }
         __syncthreads();
         int i = 0;
-        for (; i + 4 <= chunk; i += 4)
+        if (direct_phase<FP>(xo, yo, zo, c_ext))
         {
+            for (; i + 4 <= chunk; i += 4)
+            {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) term(c_e[i + u]);
+                for (int u = 0; u < 4; ++u) term(c_e[i + u], direct_t);
+            }
+            for (; i < chunk; ++i) term(c_e[i], direct_t);
         }
-        for (; i < chunk; ++i) term(c_e[i]);
+        else
+        {
+            for (; i + 4 <= chunk; i += 4)
+            {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) term(c_e[i + u], reduce_t);
+            }
+            for (; i < chunk; ++i) term(c_e[i], reduce_t);
+        }
     }
     if (!active) return;
     FP v[2 * NC];

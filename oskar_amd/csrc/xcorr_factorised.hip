@@ -162,13 +162,26 @@ __device__ __forceinline__ void gst_f4(float4* p, float x, float y, float z,
     *(GLOBAL_AS fv4*) p = v;
 }
 
-/* Splits x into hi + lo parts of a 16-bit type (round to nearest). */
+/* Splits x into hi + lo parts of a 16-bit type (round to nearest), so that
+ * hi + lo = x to the 16-bit type's precision of lo.  Both parts must come
+ * from the one rounded FP32 value x.  Where x is a product, the compiler
+ * would otherwise round the exact product to 16 bits in a mixed-precision
+ * FMA for one use of hi and round the FP32 x for another (it does so even
+ * with contraction switched off); where x sits on a 16-bit rounding tie
+ * those differ by one unit and hi + lo misses x by it (seen as a 2^-11
+ * error on the brightest source of one station).  The empty asm makes x
+ * an opaque FP32 register value; lo = x - hi is written as an FMA by an
+ * opaque 1 so that it stays one mixed-precision instruction (x - hi is
+ * exact in FP32). */
 template <typename T16>
 __device__ __forceinline__ void split16(float x, T16* hi, T16* lo)
 {
+    float one = 1.0f;
+    asm("" : "+v"(x));
+    asm("" : "+s"(one));
     const T16 h = (T16) x;
     *hi = h;
-    *lo = (T16) (x - (float) h);
+    *lo = (T16) __builtin_fmaf(x, one, -(float) h);
 }
 
 template <typename T16> __device__ __forceinline__ f16v mfma(
@@ -435,10 +448,14 @@ struct Builder
                 const float4 c = d.c[j];
                 /* W_ik = K conj(C_ik): C00, C11 real; C10 = conj(C01). */
                 float w[4][2];
-                w[0][0] = kr * c.x;  w[0][1] = ki * c.x;
-                w[1][0] = kr * c.y + ki * c.z;  w[1][1] = ki * c.y - kr * c.z;
-                w[2][0] = kr * c.y - ki * c.z;  w[2][1] = ki * c.y + kr * c.z;
-                w[3][0] = kr * c.w;  w[3][1] = ki * c.w;
+                w[0][0] = kr * c.x;
+                w[0][1] = ki * c.x;
+                w[1][0] = __builtin_fmaf(kr, c.y, ki * c.z);
+                w[1][1] = __builtin_fmaf(ki, c.y, -(kr * c.z));
+                w[2][0] = __builtin_fmaf(kr, c.y, -(ki * c.z));
+                w[2][1] = __builtin_fmaf(ki, c.y, kr * c.z);
+                w[3][0] = kr * c.w;
+                w[3][1] = ki * c.w;
 #pragma unroll
                 for (int ik = 0; ik < 4; ++ik)
                 {

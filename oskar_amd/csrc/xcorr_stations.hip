@@ -337,8 +337,11 @@ void fs_operands_kernel(int N, int S4, int SP, const FP* E, size_t ld_e,
                 for (int j = 0; j < 2; ++j)
                 {
                     const float* v = hh ? g2 : f;
-                    const float xr = v[4 * row + 2 * j];
-                    const float xi = v[4 * row + 2 * j + 1];
+                    float xr = v[4 * row + 2 * j];
+                    float xi = v[4 * row + 2 * j + 1];
+                    /* hi and lo from the one FP32 value (opaque, so that
+                     * no conversion rounds an unrounded product). */
+                    asm("" : "+v"(xr), "+v"(xi));
                     const T16 hr = (T16) xr, hi_ = (T16) xi;
                     if (!odd)
                         w[2 * hh + j] = pack2<T16>(hr, hi_);
