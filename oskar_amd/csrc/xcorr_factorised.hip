@@ -162,26 +162,33 @@ __device__ __forceinline__ void gst_f4(float4* p, float x, float y, float z,
     *(GLOBAL_AS fv4*) p = v;
 }
 
-/* Splits x into hi + lo parts of a 16-bit type (round to nearest), so that
- * hi + lo = x to the 16-bit type's precision of lo.  Both parts must come
- * from the one rounded FP32 value x.  Where x is a product, the compiler
- * would otherwise round the exact product to 16 bits in a mixed-precision
- * FMA for one use of hi and round the FP32 x for another (it does so even
- * with contraction switched off); where x sits on a 16-bit rounding tie
- * those differ by one unit and hi + lo misses x by it (seen as a 2^-11
- * error on the brightest source of one station).  The empty asm makes x
- * an opaque FP32 register value; lo = x - hi is written as an FMA by an
- * opaque 1 so that it stays one mixed-precision instruction (x - hi is
- * exact in FP32). */
+/* Splits a pair (x0, x1) into hi + lo parts of a 16-bit type (round to
+ * nearest), so that hi + lo = x to the 16-bit type's precision of lo.
+ * Both parts must come from the one rounded FP32 value x.  Where x is a
+ * product, the compiler would otherwise round the exact product to 16
+ * bits in a mixed-precision FMA for one use of hi and round the FP32 x for
+ * another (it does so even with contraction switched off); where x sits on
+ * a 16-bit rounding tie those differ by one unit and hi + lo misses x by
+ * it (seen as a 2^-11 error on the brightest source of one station).  The
+ * empty asm makes x an opaque FP32 register value.  The two hi parts come
+ * from one packed conversion, and lo = x - hi is written as an FMA by an
+ * opaque 1, so that each is one mixed-precision instruction reading its hi
+ * by halves of the packed register (x - hi is exact in FP32). */
 template <typename T16>
-__device__ __forceinline__ void split16(float x, T16* hi, T16* lo)
+__device__ __forceinline__ void split16x2(float x0, float x1, T16* hi,
+        T16* lo)
 {
+    typedef T16 t2 __attribute__((ext_vector_type(2)));
+    typedef float f2 __attribute__((ext_vector_type(2)));
     float one = 1.0f;
-    asm("" : "+v"(x));
+    asm("" : "+v"(x0), "+v"(x1));
     asm("" : "+s"(one));
-    const T16 h = (T16) x;
-    *hi = h;
-    *lo = (T16) __builtin_fmaf(x, one, -(float) h);
+    const f2 x = {x0, x1};
+    const t2 h = __builtin_convertvector(x, t2);
+    hi[0] = h.x;
+    hi[1] = h.y;
+    lo[0] = (T16) __builtin_fmaf(x0, one, -(float) h.x);
+    lo[1] = (T16) __builtin_fmaf(x1, one, -(float) h.y);
 }
 
 template <typename T16> __device__ __forceinline__ f16v mfma(
@@ -421,8 +428,7 @@ struct Builder
                 k_p(d, j, &re, &im);
                 re *= K_SCALE;
                 im *= K_SCALE;
-                split16<T16>(re, &xh[2 * j], &xl[2 * j]);
-                split16<T16>(im, &xh[2 * j + 1], &xl[2 * j + 1]);
+                split16x2<T16>(re, im, &xh[2 * j], &xl[2 * j]);
                 /* Y = (Im, -Re): negation is exact in the split parts. */
                 yh[2 * j] = xh[2 * j + 1];
                 yl[2 * j] = xl[2 * j + 1];
@@ -459,9 +465,8 @@ struct Builder
 #pragma unroll
                 for (int ik = 0; ik < 4; ++ik)
                 {
-                    split16<T16>(w[ik][0], &wh[ik][2 * j], &wl[ik][2 * j]);
-                    split16<T16>(w[ik][1], &wh[ik][2 * j + 1],
-                            &wl[ik][2 * j + 1]);
+                    split16x2<T16>(w[ik][0], w[ik][1], &wh[ik][2 * j],
+                            &wl[ik][2 * j]);
                 }
             }
 #pragma unroll
