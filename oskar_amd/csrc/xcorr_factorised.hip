@@ -219,11 +219,12 @@ __device__ __forceinline__ double phase_rev(double U, double V, double W,
     return __builtin_fma(W, n1, __builtin_fma(V, m, U * l));
 }
 
-/* exp(2 pi i r) for an FP64 phase in revolutions: reduce to |r| <= 1/2 in
- * FP64, then the hardware sine/cosine (which take revolutions). */
+/* exp(2 pi i r) for an FP64 phase in revolutions: its fractional part in
+ * [0, 1), exact in FP64 (one v_fract_f64), then the hardware sine/cosine
+ * (which take revolutions). */
 __device__ __forceinline__ void phasor(double r, float* re, float* im)
 {
-    const float rr = (float) (r - __builtin_rint(r));
+    const float rr = (float) __builtin_amdgcn_fract(r);
     *re = __builtin_amdgcn_cosf(rr);
     *im = __builtin_amdgcn_sinf(rr);
 }

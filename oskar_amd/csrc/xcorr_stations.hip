@@ -24,8 +24,8 @@
  * Three kernels plus the FP64 reduction:
  *  - sources: per source l, m, n - 1 in FP64, L_s (FP32) and D_s's signs;
  *  - operands: per (station, part of SP sources) A = K E L in FP32 (the
- *    phase U l + V m + W (n - 1) in FP64, reduced in FP64, then the
- *    hardware sine and cosine), a power-of-two scale that puts the part's
+ *    phase U l + V m + W (n - 1) in FP64, its fractional part in FP64, then
+ *    the hardware sine and cosine), a power-of-two scale that puts the part's
  *    largest component of A near 2^14, and the split into hi + lo 16-bit
  *    planes, laid out so that one station's k-step (4 sources) is one
  *    contiguous 128-byte record;
@@ -264,7 +264,7 @@ void fs_operands_kernel(int N, int S4, int SP, const FP* E, size_t ld_e,
             continue;
         const Src d = src[s];
         const double r = U * d.l + V * d.m + W * d.n1;
-        const float rr = (float) (r - __builtin_rint(r));
+        const float rr = (float) __builtin_amdgcn_fract(r);
         const float kr = __builtin_amdgcn_cosf(rr);
         const float ki = __builtin_amdgcn_sinf(rr);
         const FP* e = Ep + 8 * (size_t) s;
