@@ -505,14 +505,15 @@ __device__ __forceinline__ void store_block(float* base, int row0, int col0,
 /* tid: the thread's index in the workgroup (a value the persistent batch
  * kernel makes opaque per work item, so that nothing derived from it is
  * hoisted out of its item loop and held in registers across it). */
-template <typename T16, class TL>
+template <typename T16, class TL, bool DIAG = false>
 __device__ __forceinline__ void tile_part(const TileJob& J,
         uint4 (*lds)[TL::STAGE], int tid)
 {
     constexpr int TP = TL::TP, TW = TL::TW;
     constexpr int PS = TL::PS, QS = TL::QS, RF = TL::RF, CF = TL::CF;
     constexpr int P_ENTRIES = TL::P_ENTRIES;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const Builder<T16, TL> bld(J, tid);
     const int s_begin = J.s_begin;
     const int nsteps = (J.s_end - s_begin + KSRC - 1) / KSRC;
@@ -527,6 +528,19 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
 
     const int wp = wave % TL::WR, ww = wave / TL::WR;
     const int lr = lane & 31, lh = lane >> 5;
+    /* On a diagonal tile (P and Q the same stations) a wave's 32 x 32 block
+     * whose P stations all precede its Q stations holds no baseline: its
+     * MFMAs are skipped (the wave-uniform test below; its partial sums stay
+     * zero and are never read). */
+    auto live = [&](int rf, int cf) {
+        if constexpr (!DIAG) return true;
+        else
+        {
+            const int p_last = wp * (TP / TL::WR) + rf * 32 + 31;
+            const int q_first = (ww * (TW / TL::WC) + cf * 32) % TL::TQ;
+            return p_last >= q_first;
+        }
+    };
     StepSrc<PS, QS> cur;
     if (nsteps > 0)
     {
@@ -567,6 +581,7 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
 #pragma unroll
             for (int cf = 0; cf < CF; ++cf)
             {
+                if (!live(rf, cf)) continue;
                 re[rf][cf] = mfma<T16>(xh[rf], wh[cf], re[rf][cf]);
                 im[rf][cf] = mfma<T16>(yh[rf], wh[cf], im[rf][cf]);
             }
@@ -575,6 +590,7 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
 #pragma unroll
             for (int cf = 0; cf < CF; ++cf)
             {
+                if (!live(rf, cf)) continue;
                 re[rf][cf] = mfma<T16>(xh[rf], wl[cf], re[rf][cf]);
                 im[rf][cf] = mfma<T16>(yh[rf], wl[cf], im[rf][cf]);
             }
@@ -583,6 +599,7 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
 #pragma unroll
             for (int cf = 0; cf < CF; ++cf)
             {
+                if (!live(rf, cf)) continue;
                 re[rf][cf] = mfma<T16>(xl[rf], wh[cf], re[rf][cf]);
                 im[rf][cf] = mfma<T16>(yl[rf], wh[cf], im[rf][cf]);
             }
@@ -909,7 +926,10 @@ struct BodyOneRole
     static __device__ __forceinline__ void run(const TileJob& J,
             uint4 (*lds)[TL::STAGE], int tid)
     {
-        tile_part<T16, TL>(J, lds, tid);
+        if (TL::TP == TL::TQ && J.p0 == J.q0)
+            tile_part<T16, TL, true>(J, lds, tid);
+        else
+            tile_part<T16, TL, false>(J, lds, tid);
     }
 };
 
