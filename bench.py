@@ -749,12 +749,13 @@ def bench_dftw(args, rng):
     over the upper hemisphere, FP32, one 2x2 element pattern per direction
     (one element type), 150 MHz.  Unit: one (element, direction) term.  Its
     phases stay within 54 revolutions, so the kernel's inner loop takes the
-    path without range reduction: 6 VALU instructions (phase, four fused
-    multiply-adds) and 2 transcendental ones (sine and cosine) per term; at
-    4 cycles per wave64 instruction that bounds it at 256 CUs x 4 SIMDs x
-    64 lanes x 2.4 GHz / 32 = 4.92e12 terms/s, the roofline reported.  The
-    oracle's OpenMP body (the reference's CPU arithmetic) runs a
-    20k-direction sample on the host cores."""
+    path without range reduction: 4 VALU instructions (two fused
+    multiply-adds for the phase with the weight's phase, two into the array
+    factor with the weight's amplitude) and 2 transcendental ones (sine and
+    cosine) per term; at 4 cycles per wave64 instruction that bounds it at
+    256 CUs x 4 SIMDs x 64 lanes x 2.4 GHz / 24 = 6.55e12 terms/s, the
+    roofline reported.  The oracle's OpenMP body (the reference's CPU
+    arithmetic) runs a 20k-direction sample on the host cores."""
     import oracle
     from oskar_amd import lib
     from oskar_amd.mem import (OSKAR_COMPLEX, OSKAR_GPU, OSKAR_INT,
@@ -797,11 +798,11 @@ def bench_dftw(args, rng):
     lib.oskar_timer_free(timer)
     check_status(status, "oskar_dftw")
     rate = ne * nd / t
-    bound = 256 * 4 * 64 * 2.4e9 / 32.0
+    bound = 256 * 4 * 64 * 2.4e9 / 24.0
     res = {"workload": "4096 elements x 1M directions, FP32, one element "
                        "pattern per direction, 150 MHz",
            "ms": round(1e3 * t, 4), "terms_per_s": rate,
-           "roofline": {"bound": "valu issue (6 VALU + 2 transcendental "
+           "roofline": {"bound": "valu issue (4 VALU + 2 transcendental "
                                  "instructions per term, 4 cycles each)",
                         "achieved": rate, "peak": bound,
                         "unit": "terms/s", "frac": round(rate / bound, 3)}}

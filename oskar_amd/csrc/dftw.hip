@@ -91,7 +91,8 @@ __device__ __forceinline__ void chunk_extent(unsigned int* bits,
 
 /* Whether every phase of a thread's direction (xo, yo, zo, in revolutions
  * per metre) over elements of extent at most ext stays within the
- * hardware's exact reduction: |t| <= (|xo| + |yo| + |zo|) ext < 255. */
+ * hardware's exact reduction: |t| <= (|xo| + |yo| + |zo|) ext + 1/2 (the
+ * weight's phase) < 255. */
 template <typename FP>
 __device__ __forceinline__ bool direct_phase(FP xo, FP yo, FP zo,
         unsigned int ext_bits)
@@ -99,7 +100,70 @@ __device__ __forceinline__ bool direct_phase(FP xo, FP yo, FP zo,
     if constexpr (sizeof(FP) == 8) return false;
     else
         return (fabsf(xo) + fabsf(yo) + fabsf(zo)) *
-                __uint_as_float(ext_bits) < 255.0f;
+                __uint_as_float(ext_bits) < 254.0f;
+}
+
+/* An element's weight in the form the terms use.  FP32: amplitude and
+ * phase in revolutions, the phase then added into the element's phase by
+ * the first fused multiply-add, so that a term takes two fused
+ * multiply-adds into the array factor instead of the four of a complex
+ * product (the weight's phase in FP32 revolutions errs by under 6e-8 of a
+ * revolution, well under the FP32 phase of the element itself).  FP64:
+ * the complex weight as it is. */
+template <typename FP>
+__device__ __forceinline__ Cplx<FP> term_weight(const Cplx<FP>& w)
+{
+    if constexpr (sizeof(FP) == 4)
+    {
+        Cplx<FP> p;
+        p.x = sqrtf(w.x * w.x + w.y * w.y);
+        p.y = atan2f(w.y, w.x) * 0.159154943091895336f;    /* 1 / (2 pi) */
+        return p;
+    }
+    else
+        return w;
+}
+
+/* Phase of a term: the direction (xo, yo, zo, already in phase units per
+ * metre) dotted with the element's position, plus the FP32 weight's
+ * phase. */
+template <typename FP, bool IS_3D>
+__device__ __forceinline__ FP term_phase(FP xo, FP yo, FP zo, FP ex, FP ey,
+        FP ez, const Cplx<FP>& w)
+{
+    if constexpr (sizeof(FP) == 4)
+    {
+        float t = __builtin_fmaf(xo, ex, w.y);
+        t = __builtin_fmaf(yo, ey, t);
+        if (IS_3D) t = __builtin_fmaf(zo, ez, t);
+        return t;
+    }
+    else
+    {
+        FP t = xo * ex + yo * ey;
+        if (IS_3D) t += zo * ez;
+        return t;
+    }
+}
+
+/* af += w exp(i t), with w in term_weight's form and (s, c) the sine and
+ * cosine of the term's phase. */
+template <typename FP>
+__device__ __forceinline__ void term_add(Cplx<FP>& af, const Cplx<FP>& w,
+        FP s, FP c)
+{
+    if constexpr (sizeof(FP) == 4)
+    {
+        af.x = __builtin_fmaf(w.x, c, af.x);
+        af.y = __builtin_fmaf(w.x, s, af.y);
+    }
+    else
+    {
+        af.x += w.x * c;
+        af.x -= w.y * s;
+        af.y += w.x * s;
+        af.y += w.y * c;
+    }
 }
 
 template <typename FP, bool MATRIX, bool IS_3D>
@@ -155,16 +219,12 @@ void dftw_kernel(const int num_in, const FP wavenumber,
         }
         af.x = af.y = (FP) 0;
     };
-    /* af += w exp(i t) for element e: four fused multiply-adds. */
+    /* af += w exp(i t) for element e. */
     auto term = [&](const Elem<FP>& e, auto direct) {
-        FP t = xo * e.x + yo * e.y;
-        if (IS_3D) t += zo * e.z;
+        const FP t = term_phase<FP, IS_3D>(xo, yo, zo, e.x, e.y, e.z, e.w);
         FP s, c;
         phase_sincos<decltype(direct)::value, FP>(t, &s, &c);
-        af.x += e.w.x * c;
-        af.x -= e.w.y * s;
-        af.y += e.w.x * s;
-        af.y += e.w.y * c;
+        term_add<FP>(af, e.w, s, c);
     };
     const std::true_type direct_t;
     const std::false_type reduce_t;
@@ -187,7 +247,7 @@ void dftw_kernel(const int num_in, const FP wavenumber,
             e.y = y_in[g];
             e.z = IS_3D ? z_in[g] : (FP) 0;
             e.pad = (FP) 0;
-            e.w = weights[g];
+            e.w = term_weight<FP>(weights[g]);
             c_e[t] = e;
             chunk_extent<FP>(&c_ext, e);
             const int idx = data_idx ? data_idx[g] : g;
@@ -291,15 +351,12 @@ void aa_fused_kernel(const int num_in, const FP wavenumber,
     const int tid = (int) threadIdx.x;
     const FP kk = wavenumber * phase_unit<FP>();
 
-    /* The array factor plus w exp(i t) of one element: four fused
-     * multiply-adds, as in dftw_kernel. */
+    /* The array factor plus w exp(i t) of one element, as in dftw_kernel
+     * (w in term_weight's form). */
     auto add_term = [](Cplx<FP>& af, FP t, const Cplx<FP>& w, auto direct) {
         FP s, c;
         phase_sincos<decltype(direct)::value, FP>(t, &s, &c);
-        af.x += w.x * c;
-        af.x -= w.y * s;
-        af.y += w.x * s;
-        af.y += w.y * c;
+        term_add<FP>(af, w, s, c);
     };
     /* The beam of one direction from its array factor: pattern x af,
      * scaled, blanked (dftw_kernel's flush and output, blank_kernel). */
@@ -329,9 +386,10 @@ void aa_fused_kernel(const int num_in, const FP wavenumber,
         a.x = a.y = (FP) 0;
         for (int e = tid; e < num_in; e += THREADS)
         {
-            FP t = xn * x_in[e] + yn * y_in[e];
-            if (IS_3D) t += zn * z_in[e];
-            add_term(a, t, weights[e], std::false_type());
+            const Cplx<FP> w = term_weight<FP>(weights[e]);
+            const FP t = term_phase<FP, IS_3D>(xn, yn, zn, x_in[e], y_in[e],
+                    IS_3D ? z_in[e] : (FP) 0, w);
+            add_term(a, t, w, std::false_type());
         }
         red[tid] = a;
         __syncthreads();
@@ -371,9 +429,8 @@ void aa_fused_kernel(const int num_in, const FP wavenumber,
     Cplx<FP> af;
     af.x = af.y = (FP) 0;
     auto term = [&](const Elem<FP>& e, auto direct) {
-        FP t = xo * e.x + yo * e.y;
-        if (IS_3D) t += zo * e.z;
-        add_term(af, t, e.w, direct);
+        add_term(af, term_phase<FP, IS_3D>(xo, yo, zo, e.x, e.y, e.z, e.w),
+                e.w, direct);
     };
     const std::true_type direct_t;
     const std::false_type reduce_t;
@@ -391,7 +448,7 @@ void aa_fused_kernel(const int num_in, const FP wavenumber,
             e.y = y_in[g];
             e.z = IS_3D ? z_in[g] : (FP) 0;
             e.pad = (FP) 0;
-            e.w = weights[g];
+            e.w = term_weight<FP>(weights[g]);
             c_e[t] = e;
             chunk_extent<FP>(&c_ext, e);
         }
