@@ -333,17 +333,22 @@ struct TileJob
     float* out;
 };
 
+/* sb is a multiple of the k-step and the padded source count is too, so
+ * the slots of a step are either all valid or all past the end (a step
+ * after the last, whose build is never read): one clamp of the step's
+ * first slot to the last whole step keeps every load in bounds. */
 template <int PS, int QS>
 __device__ __forceinline__ void load_step(const TileJob& J, int sb,
         int s_last, int pg, int qg, StepSrc<PS, QS>* o)
 {
+    const int s0 = min(sb, s_last - (KSRC - 1));
 #pragma unroll
     for (int j = 0; j < PS; ++j)
-        o->p[j] = gld_src(J.src + min(sb + pg * PS + j, s_last));
+        o->p[j] = gld_src(J.src + s0 + pg * PS + j);
 #pragma unroll
     for (int j = 0; j < QS; ++j)
     {
-        const int s = min(sb + qg * QS + j, s_last);
+        const int s = s0 + qg * QS + j;
         o->w[j] = gld_src(J.src + s);
         o->c[j] = gld_f4(J.cmat + s);
     }
@@ -549,9 +554,12 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
         bld.load(J, s_begin + KSRC, &cur);
     }
     __syncthreads();
-    for (int ks = 0; ks < nsteps; ++ks)
+    /* One k-step: multiply stage st, build step ks+1 into stage nxt.  The
+     * loop below runs steps in pairs so that each stage's address is a
+     * constant of its copy of the body. */
+    auto kstep = [&](int ks, const uint4* st, uint4* nxt)
+            __attribute__((always_inline))
     {
-        const uint4* st = lds[ks & 1];
         /* Operand fragments of step ks first: the builder's LDS writes to
          * the other stage come after them in program order, so the MFMAs
          * need not wait for the build. */
@@ -606,7 +614,7 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
         if constexpr (TL::OCC >= 2) __builtin_amdgcn_s_setprio(0);
         /* Build step ks+1 (a masked dummy after the last step) into the
          * other stage, then start the loads for step ks+2. */
-        bld.build(s_begin + (ks + 1) * KSRC, cur, lds[(ks + 1) & 1]);
+        bld.build(s_begin + (ks + 1) * KSRC, cur, nxt);
         bld.load(J, s_begin + (ks + 2) * KSRC, &cur);
         if constexpr (TL::OCC >= 2)
         {
@@ -638,7 +646,14 @@ __device__ __forceinline__ void tile_part(const TileJob& J,
             __builtin_amdgcn_sched_group_barrier(0x020, 2 * PS + 3 * QS, 0);
         }
         __syncthreads();
+    };
+    int ks = 0;
+    for (; ks + 2 <= nsteps; ks += 2)
+    {
+        kstep(ks, lds[0], lds[1]);
+        kstep(ks + 1, lds[1], lds[0]);
     }
+    if (ks < nsteps) kstep(ks, lds[0], lds[1]);
 #pragma unroll
     for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
