@@ -191,6 +191,72 @@ __device__ __forceinline__ void split16x2(float x0, float x1, T16* hi,
     lo[1] = (T16) __builtin_fmaf(x1, one, -(float) h.y);
 }
 
+/* FP16: the same split in three instructions written out, so that no
+ * other instruction selection can arise: the packed conversion of the
+ * pair, then lo of each half as one mixed-precision FMA x * 1 - hi into
+ * the low and the high half of one register. */
+template <>
+__device__ __forceinline__ void split16x2<_Float16>(float x0, float x1,
+        _Float16* hi, _Float16* lo)
+{
+    unsigned int h, l;
+    asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(h) : "v"(x0), "v"(x1));
+    asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixhi_f16 %0, %3, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+        : "=&v"(l) : "v"(x0), "v"(h), "v"(x1));
+    __builtin_memcpy(hi, &h, 4);
+    __builtin_memcpy(lo, &l, 4);
+}
+
+/* split16x2 of (s x0, s x1) for a power of two s (the products are exact
+ * in FP32). */
+template <typename T16>
+__device__ __forceinline__ void split16x2_scaled(float x0, float x1,
+        float s, T16* hi, T16* lo)
+{
+    split16x2<T16>(x0 * s, x1 * s, hi, lo);
+}
+
+/* FP16: four mixed-precision FMAs, each rounding the exact product x * s
+ * (or x * s - hi) once: the same parts as the split of the scaled FP32
+ * values, without the two multiplications. */
+template <>
+__device__ __forceinline__ void split16x2_scaled<_Float16>(float x0,
+        float x1, float s, _Float16* hi, _Float16* lo)
+{
+    unsigned int h, l;
+    asm("v_fma_mixlo_f16 %0, %1, %2, 0\n\t"
+        "v_fma_mixhi_f16 %0, %3, %2, 0"
+        : "=&v"(h) : "v"(x0), "s"(s), "v"(x1));
+    asm("v_fma_mixlo_f16 %0, %1, %2, -%3 op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixhi_f16 %0, %4, %2, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+        : "=&v"(l) : "v"(x0), "s"(s), "v"(h), "v"(x1));
+    __builtin_memcpy(hi, &h, 4);
+    __builtin_memcpy(lo, &l, 4);
+}
+
+/* y = (x1, -x0) of a pair x = (x0, x1) (exact). */
+template <typename T16>
+__device__ __forceinline__ void y_pair(const T16* x, T16* y)
+{
+    y[0] = x[1];
+    y[1] = -x[0];
+}
+
+/* FP16: one packed multiply of the 32-bit word: the low result is the
+ * high half times 1, the high result the low half times -1 (1.0 in both
+ * halves of the second source, negated for the high result). */
+template <>
+__device__ __forceinline__ void y_pair<_Float16>(const _Float16* x,
+        _Float16* y)
+{
+    unsigned int w, r;
+    __builtin_memcpy(&w, x, 4);
+    asm("v_pk_mul_f16 %0, %1, %2 op_sel:[1,0] op_sel_hi:[0,1] neg_hi:[0,1]"
+        : "=v"(r) : "v"(w), "s"(0x3C003C00u));
+    __builtin_memcpy(y, &r, 4);
+}
+
 template <typename T16> __device__ __forceinline__ f16v mfma(
         const uint4& a, const uint4& b, const f16v& c);
 template <> __device__ __forceinline__ f16v mfma<_Float16>(const uint4& a,
@@ -432,14 +498,11 @@ struct Builder
             {
                 float re, im;
                 k_p(d, j, &re, &im);
-                re *= K_SCALE;
-                im *= K_SCALE;
-                split16x2<T16>(re, im, &xh[2 * j], &xl[2 * j]);
+                split16x2_scaled<T16>(re, im, K_SCALE, &xh[2 * j],
+                        &xl[2 * j]);
                 /* Y = (Im, -Re): negation is exact in the split parts. */
-                yh[2 * j] = xh[2 * j + 1];
-                yl[2 * j] = xl[2 * j + 1];
-                yh[2 * j + 1] = -xh[2 * j];
-                yl[2 * j + 1] = -xl[2 * j];
+                y_pair<T16>(&xh[2 * j], &yh[2 * j]);
+                y_pair<T16>(&xl[2 * j], &yl[2 * j]);
             }
             put<2 * PS>(&st[(0 * 2 + p_half) * TP + pb], p_off, xh);
             put<2 * PS>(&st[(1 * 2 + p_half) * TP + pb], p_off, xl);
