@@ -361,6 +361,7 @@ void fs_operands_kernel(int N, int S4, int SP, const FP* E, size_t ld_e,
 
 /* Y = (Im, -Re) per complex pair of halves: rotate by 16 bits, flip the
  * sign bit of the upper half. */
+template <typename T16>
 __device__ __forceinline__ uint4 to_y(const uint4& x)
 {
     uint4 y;
@@ -368,6 +369,27 @@ __device__ __forceinline__ uint4 to_y(const uint4& x)
     y.y = __builtin_amdgcn_alignbit(x.y, x.y, 16) ^ 0x80000000u;
     y.z = __builtin_amdgcn_alignbit(x.z, x.z, 16) ^ 0x80000000u;
     y.w = __builtin_amdgcn_alignbit(x.w, x.w, 16) ^ 0x80000000u;
+    return y;
+}
+
+/* FP16: one packed multiply per word (exact): the low result is the high
+ * half times 1, the high result the low half times -1 (1.0 in both halves
+ * of the second source, negated for the high result). */
+__device__ __forceinline__ unsigned int y_word_f16(unsigned int w)
+{
+    unsigned int r;
+    asm("v_pk_mul_f16 %0, %1, %2 op_sel:[1,0] op_sel_hi:[0,1] neg_hi:[0,1]"
+        : "=v"(r) : "v"(w), "s"(0x3C003C00u));
+    return r;
+}
+template <>
+__device__ __forceinline__ uint4 to_y<_Float16>(const uint4& x)
+{
+    uint4 y;
+    y.x = y_word_f16(x.x);
+    y.y = y_word_f16(x.y);
+    y.z = y_word_f16(x.z);
+    y.w = y_word_f16(x.w);
     return y;
 }
 
@@ -457,8 +479,8 @@ __device__ __forceinline__ void step_mfma(const uint4* stg,
 #pragma unroll
     for (int rf = 0; rf < 2; ++rf)
     {
-        yh[rf] = to_y(xh[rf]);
-        yl[rf] = to_y(xl[rf]);
+        yh[rf] = to_y<T16>(xh[rf]);
+        yl[rf] = to_y<T16>(xl[rf]);
     }
 #pragma unroll
     for (int rf = 0; rf < 2; ++rf)
