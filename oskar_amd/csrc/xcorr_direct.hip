@@ -12,8 +12,9 @@
  * Work decomposition (MI355X-specific, not the reference's):
  *  - A workgroup of 256 threads (four wave64s) owns a block of 32 "first"
  *    stations x 16 "second" stations.  Thread t owns first station t/8 and
- *    second stations t%8 and t%8 + 8, so J_first * B is formed once and
- *    reused for two baselines.
+ *    second stations t%8 and t%8 + 8.  In the matrix form M = J_first * B
+ *    is formed once per (source, first station) of a staged tile, in LDS
+ *    in place of J_first, and read by the 16 baselines that need it.
  *  - Sources are streamed in tiles of TILE sources.  The Jones values of the
  *    48 stations are loaded as 16-byte vectors, one station row of the tile
  *    per wave and instruction (64 lanes on consecutive bytes), and stored
@@ -263,6 +264,38 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
                         in ? A.c[g] : z, z};
         }
         __syncthreads();
+        if constexpr (MATRIX)
+        {
+            /* M = J_first * B once per (source, first station) of the tile,
+             * in place of J_first: B = [[I+Q, U+iV], [U-iV, I-Q]]
+             * (define_correlate_utils.h:36-39), M = J B
+             * (define_multiply.h:69-87).  Each entry is read and written
+             * by one thread. */
+            for (int i = tid; i < TILE * BLOCK_A; i += THREADS)
+            {
+                const int s = i / BLOCK_A, a = i - s * BLOCK_A;
+                FP* e = sJa[s][a];
+                const Quad<FP> fl = sFlux[s];
+                FP ja[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) ja[r] = e[r];
+                const FP Ii = fl.x, Qi = fl.y, Ui = fl.z, Vi = fl.w;
+                const FP ba = Ii + Qi, bd = Ii - Qi;
+                /* M.a = a*ba + b*conj(U+iV) */
+                e[0] = ja[0] * ba + (ja[2] * Ui + ja[3] * Vi);
+                e[1] = ja[1] * ba + (ja[3] * Ui - ja[2] * Vi);
+                /* M.b = b*bd + a*(U+iV) */
+                e[2] = ja[2] * bd + (ja[0] * Ui - ja[1] * Vi);
+                e[3] = ja[3] * bd + (ja[0] * Vi + ja[1] * Ui);
+                /* M.c = c*ba + d*conj(U+iV) */
+                e[4] = ja[4] * ba + (ja[6] * Ui + ja[7] * Vi);
+                e[5] = ja[5] * ba + (ja[7] * Ui - ja[6] * Vi);
+                /* M.d = d*bd + c*(U+iV) */
+                e[6] = ja[6] * bd + (ja[4] * Ui - ja[5] * Vi);
+                e[7] = ja[7] * bd + (ja[4] * Vi + ja[5] * Ui);
+            }
+            __syncthreads();
+        }
 
         FP t[2][NR];
 #pragma unroll
@@ -290,23 +323,10 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
             const FP* ja = sJa[s][la];
             if (MATRIX)
             {
-                /* B = [[I+Q, U+iV], [U-iV, I-Q]] (define_correlate_utils.h:36-39);
-                 * M = J_first * B (define_multiply.h:69-87). */
-                const FP Ii = fl.x, Qi = fl.y, Ui = fl.z, Vi = fl.w;
-                const FP ba = Ii + Qi, bd = Ii - Qi;
+                /* The tile holds M = J_first * B (formed after staging). */
                 FP M[8];
-                /* M.a = a*ba + b*conj(U+iV) */
-                M[0] = ja[0] * ba + (ja[2] * Ui + ja[3] * Vi);
-                M[1] = ja[1] * ba + (ja[3] * Ui - ja[2] * Vi);
-                /* M.b = b*bd + a*(U+iV) */
-                M[2] = ja[2] * bd + (ja[0] * Ui - ja[1] * Vi);
-                M[3] = ja[3] * bd + (ja[0] * Vi + ja[1] * Ui);
-                /* M.c = c*ba + d*conj(U+iV) */
-                M[4] = ja[4] * ba + (ja[6] * Ui + ja[7] * Vi);
-                M[5] = ja[5] * ba + (ja[7] * Ui - ja[6] * Vi);
-                /* M.d = d*bd + c*(U+iV) */
-                M[6] = ja[6] * bd + (ja[4] * Ui - ja[5] * Vi);
-                M[7] = ja[7] * bd + (ja[4] * Vi + ja[5] * Ui);
+#pragma unroll
+                for (int r = 0; r < 8; ++r) M[r] = ja[r];
 #pragma unroll
                 for (int k = 0; k < 2; ++k)
                 {
