@@ -124,13 +124,20 @@ __device__ __forceinline__ FP smearing(const BaselineTerms<FP, TS>& t,
  * value is moved as VPE vectors of VR reals (16 bytes, or 8 for a float
  * complex scalar); TILE * VPE = 64, so one wave loads one station row of
  * a tile with consecutive lanes on consecutive bytes. */
-template <typename FP, bool MATRIX> struct Geometry
+/* PLAIN: point sources without smearing.  The FP32 matrix kernel of that
+ * case stages 16 sources per tile (114 VGPRs, 26 KB of LDS: four waves per
+ * SIMD); with a smearing or Gaussian factor it stages 32 (fewer
+ * barriers per source, measured faster there at three waves per SIMD). */
+template <typename FP, bool MATRIX, bool PLAIN> struct Geometry
 {
     static constexpr int NR = MATRIX ? 8 : 2;
-    static constexpr int TILE = MATRIX ? (sizeof(FP) == 4 ? 32 : 16) : 64;
+    static constexpr int TILE = MATRIX ?
+            ((sizeof(FP) == 4 && !PLAIN) ? 32 : 16) : 64;
     static constexpr int VPE = MATRIX ? (sizeof(FP) == 4 ? 2 : 4) : 1;
     static constexpr int VR = NR / VPE;
-    static_assert(TILE * VPE == 64, "one wave per station row of a tile");
+    /* Station rows of a tile moved by one wave instruction. */
+    static constexpr int RPI = 64 / (TILE * VPE);
+    static_assert(RPI * TILE * VPE == 64, "whole station rows per wave");
 };
 
 template <typename FP, int VR> struct alignas(VR * sizeof(FP)) Vec
@@ -148,7 +155,7 @@ template <typename FP, bool MATRIX, bool GAUSS, bool BS, bool TS>
 __global__ __launch_bounds__(THREADS)
 void xcorr_direct_kernel(XcorrArgs<FP> A)
 {
-    using G = Geometry<FP, MATRIX>;
+    using G = Geometry<FP, MATRIX, !(GAUSS || BS || TS)>;
     constexpr int NR = G::NR;
     constexpr int TILE = G::TILE;
     constexpr int VPE = G::VPE;
@@ -209,13 +216,16 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
         for (int r = 0; r < NR; ++r) acc[k][r] = 0.0;
 
     const FP* J = (const FP*) A.jones;
-    /* Tile loads: wave w moves station rows w, w + 4, ... of the block
-     * (rows 0-31 first stations, 32-47 second stations); lane (s, q) moves
-     * vector q of source s.  Whole-vector loads need the Jones array
-     * aligned to a vector, which element-aligned arrays are; otherwise the
-     * same mapping moves one real at a time. */
+    /* Tile loads: instruction i of wave w moves station rows
+     * (w + 4 i) RPI + r, r < RPI, of the block (rows 0-31 first stations,
+     * 32-47 second stations); lane (r, s, q) moves vector q of source s of
+     * row r.  Whole-vector loads need the Jones array aligned to a vector,
+     * which element-aligned arrays are; otherwise the same mapping moves
+     * one real at a time. */
+    constexpr int RPI = G::RPI;
     const int lane = tid & 63, wave = tid >> 6;
-    const int lq = lane % VPE, ls = lane / VPE;
+    const int lq = lane % VPE, ls = (lane / VPE) % TILE;
+    const int lr = lane / (TILE * VPE);
     const bool vec_ok = (((uintptr_t) J) % sizeof(V)) == 0;
     for (int s0 = s_begin; s0 < s_end; s0 += TILE)
     {
@@ -223,10 +233,10 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
         __syncthreads();
         const bool s_ok = ls < ns;
 #pragma unroll
-        for (int i = 0; i < (BLOCK_A + BLOCK_B) / 4; ++i)
+        for (int i = 0; i < (BLOCK_A + BLOCK_B) / (4 * RPI); ++i)
         {
-            const int st = wave + 4 * i;
-            const bool first = i < BLOCK_A / 4;
+            const int st = (wave + 4 * i) * RPI + lr;
+            const bool first = st < BLOCK_A;
             const int station = first ? a0 + st : b0 + (st - BLOCK_A);
             FP* dst = first ? &sJa[ls][st][lq * VR] :
                     &sJb[ls][st - BLOCK_A][lq * VR];
@@ -473,7 +483,7 @@ hipError_t scratch_address(double** out)
 template <typename FP, bool MATRIX, bool GAUSS, bool BS, bool TS>
 hipError_t launch_t(XcorrArgs<FP> A, hipStream_t stream, double* fixed)
 {
-    using G = Geometry<FP, MATRIX>;
+    using G = Geometry<FP, MATRIX, !(GAUSS || BS || TS)>;
     const int N = A.num_stations;
     const int na = (N + BLOCK_A - 1) / BLOCK_A;
     const int nbk = (N + BLOCK_B - 1) / BLOCK_B;
