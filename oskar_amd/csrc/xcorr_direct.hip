@@ -124,15 +124,14 @@ __device__ __forceinline__ FP smearing(const BaselineTerms<FP, TS>& t,
  * value is moved as VPE vectors of VR reals (16 bytes, or 8 for a float
  * complex scalar); TILE * VPE = 64, so one wave loads one station row of
  * a tile with consecutive lanes on consecutive bytes. */
-/* PLAIN: point sources without smearing.  The FP32 matrix kernel of that
- * case stages 16 sources per tile (114 VGPRs, 26 KB of LDS: four waves per
- * SIMD); with a smearing or Gaussian factor it stages 32 (fewer
- * barriers per source, measured faster there at three waves per SIMD). */
-template <typename FP, bool MATRIX, bool PLAIN> struct Geometry
+/* Matrix kernels stage 16 sources per tile: with the FP32 inner loop
+ * taking two sources per iteration for point sources without smearing
+ * and one otherwise, every FP32 matrix variant fits 128 VGPRs and 26 KB of
+ * LDS, four waves per SIMD. */
+template <typename FP, bool MATRIX> struct Geometry
 {
     static constexpr int NR = MATRIX ? 8 : 2;
-    static constexpr int TILE = MATRIX ?
-            ((sizeof(FP) == 4 && !PLAIN) ? 32 : 16) : 64;
+    static constexpr int TILE = MATRIX ? 16 : 64;
     static constexpr int VPE = MATRIX ? (sizeof(FP) == 4 ? 2 : 4) : 1;
     static constexpr int VR = NR / VPE;
     /* Station rows of a tile moved by one wave instruction. */
@@ -155,7 +154,7 @@ template <typename FP, bool MATRIX, bool GAUSS, bool BS, bool TS>
 __global__ __launch_bounds__(THREADS)
 void xcorr_direct_kernel(XcorrArgs<FP> A)
 {
-    using G = Geometry<FP, MATRIX, !(GAUSS || BS || TS)>;
+    using G = Geometry<FP, MATRIX>;
     constexpr int NR = G::NR;
     constexpr int TILE = G::TILE;
     constexpr int VPE = G::VPE;
@@ -313,9 +312,11 @@ void xcorr_direct_kernel(XcorrArgs<FP> A)
 #pragma unroll
             for (int r = 0; r < NR; ++r) t[k][r] = (FP) 0;
 
-        /* Two sources per iteration in FP32; one in FP64, whose registers
-         * would otherwise limit the kernel to two waves per SIMD. */
-        constexpr int UNROLL = sizeof(FP) == 4 ? 2 : 1;
+        /* Two sources per iteration in FP32, except in the matrix form
+         * with a smearing or Gaussian factor: one there (the factor's
+         * registers would leave three waves per SIMD); one in FP64. */
+        constexpr int UNROLL = (sizeof(FP) == 4 &&
+                (!MATRIX || !(GAUSS || BS || TS))) ? 2 : 1;
 #pragma unroll UNROLL
         for (int s = 0; s < TILE; ++s)
         {
@@ -483,7 +484,7 @@ hipError_t scratch_address(double** out)
 template <typename FP, bool MATRIX, bool GAUSS, bool BS, bool TS>
 hipError_t launch_t(XcorrArgs<FP> A, hipStream_t stream, double* fixed)
 {
-    using G = Geometry<FP, MATRIX, !(GAUSS || BS || TS)>;
+    using G = Geometry<FP, MATRIX>;
     const int N = A.num_stations;
     const int na = (N + BLOCK_A - 1) / BLOCK_A;
     const int nbk = (N + BLOCK_B - 1) / BLOCK_B;
